@@ -1,0 +1,37 @@
+"""gamdpy_amd: gamdpy's per-timestep MD path on AMD Instinct MI355X (gfx950).
+
+Drop-in for the part of the gamdpy API that drives the timestep loop:
+
+    import gamdpy_amd as gp
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[8, 8, 8], rho=0.8442)
+    conf['m'] = 1.0
+    conf.randomize_velocities(temperature=1.44, seed=0)
+    pair_pot = gp.PairPotential(gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon),
+                                params=[1.0, 1.0, 2.5], max_num_nbs=500)
+    sim = gp.Simulation(conf, pair_pot, gp.integrators.NVE(dt=0.005), [gp.MomentumReset(100)],
+                        num_timeblocks=4, steps_per_timeblock=1024, storage='memory')
+    for block in sim.run_timeblocks():
+        print(sim.status(per_particle=True))
+
+The compute runs in hand-written HIP kernels (libgamdpy_amd.so) reached
+through the C ABI in include/gamdpy_amd.h.
+"""
+from . import configuration as _configuration_module
+from .configuration import Configuration, colarray, make_lattice, unit_cells, generate_random_velocities
+from .configuration import get_default_compute_flags
+from .simulation_boxes import Orthorhombic
+from .potential_functions import (LJ_12_6, LJ_12_6_sigma_epsilon, apply_shifted_potential_cutoff,
+                                  apply_shifted_force_cutoff)
+from .interactions import Interaction, PairPotential, NbList2, NbListLinkedLists
+from . import integrators
+from .integrators import Integrator, NVE, NVT, NVT_Langevin
+from .runtime_actions import RuntimeAction, MomentumReset, ScalarSaver
+from .simulation import Simulation, get_default_compute_plan
+from .calculators import CalculatorRadialDistribution
+from .evaluator import Evaluator
+from .misc import make_function_constant, make_function_ramp, make_function_sin, extract_scalars
+
+configuration = _configuration_module
+
+__version__ = "0.1.0"

@@ -1,0 +1,305 @@
+"""ctypes binding of the C ABI declared in include/gamdpy_amd.h.
+
+The shared library ``libgamdpy_amd.so`` is built in-tree by
+``gamdpy_amd._build.build()``.  Nothing here falls back to a CPU
+implementation: if the library is missing or the GPU is absent, the call
+that needs it raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgamdpy_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gamdpy_amd.h")
+
+ABI_VERSION = 3
+
+# enums of gamdpy_amd.h
+PAIR_BASE = {"LJ_12_6_sigma_epsilon": 0, "LJ_12_6": 1}
+CUTOFF = {None: 0, "SP": 1, "SF": 2}
+NBLIST = {"N squared": 0, "linked lists": 1}
+INTEGRATOR = {None: 0, "NVE": 1, "NVT": 2, "NVT_Langevin": 3}
+TFUNC = {"constant": 0, "ramp": 1, "sin": 2}
+SAVE_COLUMNS = ["U", "W", "lapU", "K", "Fsq", "Vol"]
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_u64_p = ctypes.POINTER(ctypes.c_ulonglong)
+
+
+class GmdDesc(ctypes.Structure):
+    """Mirror of ``gmd_desc`` (field order and types must match the header)."""
+    _fields_ = [
+        ("abi_version", ctypes.c_int),
+        ("N", ctypes.c_int), ("D", ctypes.c_int), ("ntypes", ctypes.c_int),
+        ("box", ctypes.c_float * 3),
+        ("pair_base", ctypes.c_int), ("cutoff", ctypes.c_int),
+        ("params", c_float_p),
+        ("max_cut", ctypes.c_float),
+        ("nblist_kind", ctypes.c_int),
+        ("skin", ctypes.c_float),
+        ("max_nbs", ctypes.c_int),
+        ("utilize_niii", ctypes.c_int),
+        ("exclusions", c_int_p),
+        ("exclusions_cols", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("dt", ctypes.c_float),
+        ("omega2", ctypes.c_float),
+        ("degrees", ctypes.c_longlong),
+        ("alpha", ctypes.c_float),
+        ("seed", ctypes.c_ulonglong),
+        ("tfunc", ctypes.c_int),
+        ("tparams", ctypes.c_float * 4),
+        ("compute_u", ctypes.c_int), ("compute_w", ctypes.c_int), ("compute_lap", ctypes.c_int),
+        ("compute_k", ctypes.c_int), ("compute_fsq", ctypes.c_int),
+        ("nscalars", ctypes.c_int), ("sid_u", ctypes.c_int), ("sid_w", ctypes.c_int), ("sid_lap", ctypes.c_int),
+        ("sid_k", ctypes.c_int), ("sid_fsq", ctypes.c_int), ("sid_m", ctypes.c_int),
+        ("momentum_reset_every", ctypes.c_int),
+        ("scalar_save_every", ctypes.c_int),
+        ("scalar_saves_per_block", ctypes.c_int),
+        ("pb", ctypes.c_int), ("tp", ctypes.c_int), ("gridsync", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+    ]
+
+
+class GmdDeviceInfo(ctypes.Structure):
+    _fields_ = [("compute_units", ctypes.c_int), ("wavefront_size", ctypes.c_int),
+                ("max_threads_per_block", ctypes.c_int), ("total_mem", ctypes.c_longlong),
+                ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 64)]
+
+
+_SIGNATURES = {
+    "gmd_abi_version": (ctypes.c_int, []),
+    "gmd_desc_size": (ctypes.c_int, []),
+    "gmd_last_error": (ctypes.c_char_p, []),
+    "gmd_get_device_info": (ctypes.c_int, [ctypes.POINTER(GmdDeviceInfo)]),
+    "gmd_create": (ctypes.c_int, [ctypes.POINTER(GmdDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "gmd_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_upload": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_float_p, c_int_p, c_int_p]),
+    "gmd_download": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_float_p, c_int_p, c_int_p]),
+    "gmd_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]),
+    "gmd_evaluate": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "gmd_read_nbflag": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
+    "gmd_read_nblist": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
+    "gmd_force_rebuild": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_get_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "gmd_set_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "gmd_read_scalar_output": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "gmd_rdf_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "gmd_rdf_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_rdf_compute_ctx": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int_p, c_u64_p]),
+    "gmd_rdf_compute_host": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_int_p, c_float_p, c_u64_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class NativeError(RuntimeError):
+    """A call into libgamdpy_amd.so failed (message from gmd_last_error)."""
+
+
+def library_path():
+    return LIB_PATH
+
+
+def load():
+    """Load libgamdpy_amd.so (once) and attach the signatures; raises if absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `python -m gamdpy_amd._build` (there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.gmd_abi_version() != ABI_VERSION:
+            raise NativeError(f"libgamdpy_amd ABI {lib.gmd_abi_version()} != binding ABI {ABI_VERSION}")
+        if lib.gmd_desc_size() != ctypes.sizeof(GmdDesc):
+            raise NativeError(f"gmd_desc size mismatch: library {lib.gmd_desc_size()}, binding {ctypes.sizeof(GmdDesc)}")
+        _lib = lib
+        return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = _lib.gmd_last_error().decode(errors="replace") if _lib is not None else "library not loaded"
+        raise NativeError(f"{what}: {msg}" if what else msg)
+
+
+def fptr(a):
+    return a.ctypes.data_as(c_float_p) if a is not None else None
+
+
+def iptr(a):
+    return a.ctypes.data_as(c_int_p) if a is not None else None
+
+
+def u64ptr(a):
+    return a.ctypes.data_as(c_u64_p)
+
+
+_device_info = None
+
+
+def device_info():
+    """Compute units, wavefront size and name of the current HIP device."""
+    global _device_info
+    if _device_info is None:
+        lib = load()
+        info = GmdDeviceInfo()
+        check(lib.gmd_get_device_info(ctypes.byref(info)), "gmd_get_device_info")
+        _device_info = {"compute_units": info.compute_units, "wavefront_size": info.wavefront_size,
+                        "max_threads_per_block": info.max_threads_per_block, "total_mem": info.total_mem,
+                        "name": info.name.decode(), "arch": info.arch.decode()}
+    return _device_info
+
+
+class Context:
+    """Owns one gmd_ctx: the device state of one Simulation (or Evaluator)."""
+
+    def __init__(self, desc_fields, params, exclusions=None):
+        self.lib = load()
+        self._params = np.ascontiguousarray(params, dtype=np.float32)
+        self._excl = None if exclusions is None else np.ascontiguousarray(exclusions, dtype=np.int32)
+        d = GmdDesc()
+        d.abi_version = ABI_VERSION
+        for k, v in desc_fields.items():
+            if k == "box":
+                d.box = (ctypes.c_float * 3)(*[float(x) for x in v])
+            elif k == "tparams":
+                d.tparams = (ctypes.c_float * 4)(*[float(x) for x in v])
+            else:
+                setattr(d, k, v)
+        d.params = fptr(self._params)
+        if self._excl is not None:
+            d.exclusions = iptr(self._excl)
+            d.exclusions_cols = self._excl.shape[1]
+        self.desc = d
+        self.N = d.N
+        self.nscalars = d.nscalars
+        self.max_nbs = d.max_nbs
+        self.saves_per_block = d.scalar_saves_per_block
+        h = ctypes.c_void_p()
+        check(self.lib.gmd_create(ctypes.byref(d), ctypes.byref(h)), "gmd_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gmd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, r=None, v=None, f=None, scalars=None, ptype=None, r_im=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=dt)
+                for a, dt in ((r, np.float32), (v, np.float32), (f, np.float32), (scalars, np.float32),
+                              (ptype, np.int32), (r_im, np.int32))]
+        check(self.lib.gmd_upload(self.h, fptr(arrs[0]), fptr(arrs[1]), fptr(arrs[2]), fptr(arrs[3]),
+                                  iptr(arrs[4]), iptr(arrs[5])), "gmd_upload")
+
+    def download(self):
+        N = self.N
+        r = np.zeros((N, 3), np.float32)
+        v = np.zeros((N, 3), np.float32)
+        f = np.zeros((N, 3), np.float32)
+        sc = np.zeros((N, self.nscalars), np.float32)
+        pt = np.zeros(N, np.int32)
+        im = np.zeros((N, 3), np.int32)
+        check(self.lib.gmd_download(self.h, fptr(r), fptr(v), fptr(f), fptr(sc), iptr(pt), iptr(im)), "gmd_download")
+        return r, v, f, sc, pt, im
+
+    def run(self, time_zero, steps):
+        check(self.lib.gmd_run(self.h, ctypes.c_float(np.float32(time_zero)), int(steps)), "gmd_run")
+
+    def evaluate(self):
+        check(self.lib.gmd_evaluate(self.h), "gmd_evaluate")
+
+    def last_run_ms(self):
+        x = ctypes.c_float()
+        check(self.lib.gmd_last_run_ms(self.h, ctypes.byref(x)), "gmd_last_run_ms")
+        return float(x.value)
+
+    def nbflag(self):
+        a = np.zeros(3, np.int32)
+        check(self.lib.gmd_read_nbflag(self.h, iptr(a)), "gmd_read_nbflag")
+        return a
+
+    def nblist(self):
+        a = np.zeros((self.N, self.max_nbs + 1), np.int32)
+        check(self.lib.gmd_read_nblist(self.h, iptr(a)), "gmd_read_nblist")
+        return a
+
+    def force_rebuild(self):
+        check(self.lib.gmd_force_rebuild(self.h), "gmd_force_rebuild")
+
+    def get_thermostat(self):
+        a = np.zeros(2, np.float32)
+        check(self.lib.gmd_get_thermostat(self.h, fptr(a)), "gmd_get_thermostat")
+        return a
+
+    def set_thermostat(self, state):
+        a = np.ascontiguousarray(state, dtype=np.float32)
+        check(self.lib.gmd_set_thermostat(self.h, fptr(a)), "gmd_set_thermostat")
+
+    def scalar_output(self):
+        a = np.zeros((self.saves_per_block, len(SAVE_COLUMNS)), np.float32)
+        check(self.lib.gmd_read_scalar_output(self.h, fptr(a)), "gmd_read_scalar_output")
+        return a
+
+
+class RdfHandle:
+    """Owns one gmd_rdf (radial-distribution histogram kernel)."""
+
+    def __init__(self, N, ntypes, bins):
+        self.lib = load()
+        self.N, self.ntypes, self.bins = int(N), int(ntypes), int(bins)
+        h = ctypes.c_void_p()
+        check(self.lib.gmd_rdf_create(self.N, self.ntypes, self.bins, ctypes.byref(h)), "gmd_rdf_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gmd_rdf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute_ctx(self, ctx, ptype=None):
+        out = np.zeros((self.ntypes, self.ntypes, self.bins), np.uint64)
+        pt = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.int32)
+        check(self.lib.gmd_rdf_compute_ctx(self.h, ctx.h, iptr(pt), u64ptr(out)), "gmd_rdf_compute_ctx")
+        return out
+
+    def compute_host(self, r, ptype, box):
+        out = np.zeros((self.ntypes, self.ntypes, self.bins), np.uint64)
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        pt = np.ascontiguousarray(ptype, dtype=np.int32)
+        bx = np.ascontiguousarray(box, dtype=np.float32)
+        check(self.lib.gmd_rdf_compute_host(self.h, fptr(r), iptr(pt), fptr(bx), u64ptr(out)), "gmd_rdf_compute_host")
+        return out
+
+
+def header_functions(path=HEADER_PATH):
+    """Names of the gmd_* functions declared in include/gamdpy_amd.h."""
+    import re
+    txt = open(path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gmd_[a-z0-9_]+)\s*\(", txt)))

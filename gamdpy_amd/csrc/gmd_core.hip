@@ -1,0 +1,1080 @@
+// gmd_core.hip -- kernels, launch drivers and the C ABI of gamdpy_amd.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gamdpy_amd.h"
+#include "gmd_kernels.h"
+
+// ============================================================================
+// Kernels: persistent (cooperative) driver
+// ============================================================================
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(1024) k_persistent(DP P) {
+    extern __shared__ __align__(16) char lds[];
+    PairP *pp = (PairP *)(lds + P.lds_pp);
+    double *red = (double *)(lds + P.lds_red);
+    char *build = lds + P.lds_build;
+    load_pair_table(P, pp);
+    __syncthreads();
+    const int nsteps = P.ctrl->nsteps;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gstride = gridDim.x * blockDim.x;
+    const int wpb = blockDim.x >> 6;
+    const int wave_global = blockIdx.x * wpb + (threadIdx.x >> 6);
+    const int wave_stride = gridDim.x * wpb;
+    for (int s = 0; s <= nsteps; ++s) {
+        if (ld_flag(&P.rebuild_flag[s])) {
+            const float4 *pos = cur_pos(P, s);
+            if (P.nblist_kind == 1) {
+                ll_assign(P, pos, gtid, gstride);
+                grid_barrier(P.ctrl, NoTail());
+                if (blockIdx.x == 0) ll_scan_block(P, (int *)build);
+                grid_barrier(P.ctrl, NoTail());
+                ll_scatter(P, gtid, gstride);
+                grid_barrier(P.ctrl, NoTail());
+                ll_sort_cells(P, gtid, gstride);
+                grid_barrier(P.ctrl, NoTail());
+                ll_build(P, pos, build, wave_global, wave_stride);
+            } else {
+                nsq_build(P, pos, (float4 *)build);
+            }
+            if (gtid == 0) P.ctrl->nbflag[2] += 1;
+            grid_barrier(P.ctrl, NoTail());
+        }
+        const StepInfo I = step_info(P, s, INTEG);
+        float acc[GMD_NRED];
+#pragma unroll
+        for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
+        phase_step<BASE, CUT, INTEG>(P, I, pp, acc);
+        if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, acc, red);
+        grid_barrier(P.ctrl, [&]() { step_tail(P, I, red); });
+    }
+}
+
+// ============================================================================
+// Kernels: kernel-per-phase driver (each reads the step from the control block)
+// ============================================================================
+__device__ __forceinline__ bool rebuild_active(const DP &P, int &s) {
+    s = P.ctrl->step;
+    if (s > P.ctrl->nsteps) return false;
+    return ld_flag(&P.rebuild_flag[s]) != 0;
+}
+
+__global__ void k_ll_assign(DP P) {
+    int s;
+    if (!rebuild_active(P, s)) return;
+    ll_assign(P, cur_pos(P, s), blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(1024) k_ll_scan(DP P) {
+    extern __shared__ __align__(16) char lds[];
+    int s;
+    if (!rebuild_active(P, s)) return;
+    ll_scan_block(P, (int *)lds);
+}
+
+__global__ void k_ll_scatter(DP P) {
+    int s;
+    if (!rebuild_active(P, s)) return;
+    ll_scatter(P, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void k_ll_sort(DP P) {
+    int s;
+    if (!rebuild_active(P, s)) return;
+    ll_sort_cells(P, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(1024) k_ll_build(DP P) {
+    extern __shared__ __align__(16) char lds[];
+    int s;
+    if (!rebuild_active(P, s)) return;
+    const int wpb = blockDim.x >> 6;
+    ll_build(P, cur_pos(P, s), lds, blockIdx.x * wpb + (threadIdx.x >> 6), gridDim.x * wpb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&P.ctrl->nbflag[2], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_nsq_build(DP P) {
+    extern __shared__ __align__(16) char lds[];
+    int s;
+    if (!rebuild_active(P, s)) return;
+    nsq_build(P, cur_pos(P, s), (float4 *)lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&P.ctrl->nbflag[2], 1);
+}
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(1024) k_step(DP P) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ int s_last;
+    const int s = P.ctrl->step;
+    if (s > P.ctrl->nsteps) return;
+    PairP *pp = (PairP *)(lds + P.lds_pp);
+    double *red = (double *)(lds + P.lds_red);
+    load_pair_table(P, pp);
+    __syncthreads();
+    const StepInfo I = step_info(P, s, INTEG);
+    float acc[GMD_NRED];
+#pragma unroll
+    for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
+    phase_step<BASE, CUT, INTEG>(P, I, pp, acc);
+    if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, acc, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned a = atomicAdd(&P.ctrl->red_count, 1u);
+        s_last = (a == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        step_tail(P, I, red);
+        if (threadIdx.x == 0) {
+            P.ctrl->red_count = 0;
+            P.ctrl->step = s + 1;
+            __threadfence();
+        }
+    }
+}
+
+// Skin test of the positions a run starts from (nblist.py:58-86 at step 0):
+// a rebuild is due when any particle is more than skin/2 from r_ref.
+__global__ void k_skin_check(DP P) {
+    const float4 *pos = cur_pos(P, 0);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += gridDim.x * blockDim.x) {
+        const float4 r = pos[i], rr = P.rref[i];
+        float dx, dy, dz;
+        if (dist_sq_mic(r.x, r.y, r.z, rr.x, rr.y, rr.z, P.box0, P.box1, P.box2, dx, dy, dz) > P.skin2q)
+            P.rebuild_flag[0] = 1;
+    }
+}
+
+__global__ void k_begin_run(Ctrl *C, int nsteps, float t0, int buf0, long long lstep0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        C->nsteps = nsteps;
+        C->t0 = t0;
+        C->buf0 = buf0;
+        C->lstep0 = lstep0;
+        C->step = 0;
+        C->red_count = 0;
+        C->bar_count = 0;
+    }
+}
+
+// ============================================================================
+// Radial distribution (calculator_radial_distribution.py:76-109)
+// ============================================================================
+struct RdfP {
+    int N, nt, bins, nclass, tile, ntiles;
+    long long npairs;
+    float L0, L1, L2, inv_bw;
+    const float4 *pos;
+    const int *ptype;   // nullptr: type from pos.w
+    const float *thr2;  // bins+1 thresholds on the squared distance
+    unsigned *part;     // [gridDim][nclass*bins]
+};
+
+__device__ __forceinline__ int type_class(int a, int b, int nt) {
+    if (a > b) {
+        int t = a;
+        a = b;
+        b = t;
+    }
+    return a * nt - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Unordered pairs i < j are visited once; each adds one count to the class
+// {ti, tj}.  Ordered counts are recovered on the host (the MIC distance is
+// symmetric bit for bit, so d(i,j) == d(j,i)).
+__global__ void __launch_bounds__(256) k_rdf(RdfP R) {
+    extern __shared__ __align__(16) char lds[];
+    const int nh = R.nclass * R.bins;
+    unsigned *hist = (unsigned *)lds;
+    float *thr = (float *)(hist + nh);
+    float4 *tj = (float4 *)(lds + (((size_t)nh * 4 + (size_t)(R.bins + 1) * 4 + 15) & ~(size_t)15));
+    int *tt = (int *)(tj + R.tile);
+    for (int x = threadIdx.x; x < nh; x += blockDim.x) hist[x] = 0u;
+    for (int x = threadIdx.x; x <= R.bins; x += blockDim.x) thr[x] = R.thr2[x];
+    __syncthreads();
+    const float out_thr = thr[R.bins];
+    const long long T = R.ntiles;
+    for (long long p = blockIdx.x; p < R.npairs; p += gridDim.x) {
+        // decode p -> (I, J), J >= I, rows of length T - I
+        double a = (double)(2 * T + 1);
+        long long I = (long long)((a - sqrt(a * a - 8.0 * (double)p)) * 0.5);
+        if (I < 0) I = 0;
+        if (I > T - 1) I = T - 1;
+        while (I > 0 && I * T - (I * (I - 1)) / 2 > p) --I;
+        while (I + 1 < T && (I + 1) * T - ((I + 1) * I) / 2 <= p) ++I;
+        const long long J = I + (p - (I * T - (I * (I - 1)) / 2));
+        __syncthreads();
+        const int j0 = (int)(J * R.tile);
+        for (int x = threadIdx.x; x < R.tile; x += blockDim.x) {
+            int j = j0 + x;
+            if (j < R.N) {
+                tj[x] = R.pos[j];
+                tt[x] = R.ptype ? R.ptype[j] : __float_as_int(R.pos[j].w);
+            }
+        }
+        __syncthreads();
+        const int i = (int)(I * R.tile) + threadIdx.x;
+        if (threadIdx.x < R.tile && i < R.N) {
+            const float4 ri = R.pos[i];
+            const int ti = R.ptype ? R.ptype[i] : __float_as_int(ri.w);
+            const int jn = min(R.tile, R.N - j0);
+            const int jstart = (I == J) ? (threadIdx.x + 1) : 0;
+            for (int jj = jstart; jj < jn; ++jj) {
+                const float4 rj = tj[jj];
+                float dx, dy, dz;
+                const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, R.L0, R.L1, R.L2, dx, dy, dz);
+                if (d2 < out_thr) {
+                    int k = (int)(sqrtf(d2) * R.inv_bw);
+                    k = max(0, min(k, R.bins - 1));
+                    while (k > 0 && d2 < thr[k]) --k;
+                    while (k + 1 < R.bins && d2 >= thr[k + 1]) ++k;
+                    atomicAdd(&hist[type_class(ti, tt[jj], R.nt) * R.bins + k], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    unsigned *dst = R.part + (size_t)blockIdx.x * nh;
+    for (int x = threadIdx.x; x < nh; x += blockDim.x) dst[x] = hist[x];
+}
+
+__global__ void k_rdf_reduce(const unsigned *part, int nblocks, int nh, unsigned long long *out) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nh) return;
+    unsigned long long s = 0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nh + x];
+    out[x] = s;
+}
+
+__global__ void k_pack_pos(const float *r, const int *ptype, int N, float4 *pos) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) pos[i] = make_float4(r[3 * i], r[3 * i + 1], r[3 * i + 2], __int_as_float(ptype[i]));
+}
+
+// ============================================================================
+// Host side
+// ============================================================================
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return -1;
+}
+
+#define HIPCHECK(x)                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail("%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHECK(hipMalloc((void **)p, n * sizeof(T)));
+    return 0;
+}
+
+typedef void (*kfun)(DP);
+
+template <int BASE, int CUT, int INTEG>
+void pick(kfun &pers, kfun &step) {
+    pers = k_persistent<BASE, CUT, INTEG>;
+    step = k_step<BASE, CUT, INTEG>;
+}
+
+template <int BASE, int CUT>
+void pick_integ(int integ, kfun &pers, kfun &step) {
+    switch (integ) {
+        case 0: pick<BASE, CUT, 0>(pers, step); break;
+        case 1: pick<BASE, CUT, 1>(pers, step); break;
+        case 2: pick<BASE, CUT, 2>(pers, step); break;
+        default: pick<BASE, CUT, 3>(pers, step); break;
+    }
+}
+
+template <int BASE>
+void pick_cut(int cut, int integ, kfun &pers, kfun &step) {
+    switch (cut) {
+        case 0: pick_integ<BASE, 0>(integ, pers, step); break;
+        case 1: pick_integ<BASE, 1>(integ, pers, step); break;
+        default: pick_integ<BASE, 2>(integ, pers, step); break;
+    }
+}
+
+void pick_kernels(int base, int cut, int integ, kfun &pers, kfun &step) {
+    if (base == 0)
+        pick_cut<0>(cut, integ, pers, step);
+    else
+        pick_cut<1>(cut, integ, pers, step);
+}
+
+float i2f(int v) {
+    float f;
+    memcpy(&f, &v, 4);
+    return f;
+}
+
+int f2i(float v) {
+    int i;
+    memcpy(&i, &v, 4);
+    return i;
+}
+
+bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+
+int ilog2(int x) {
+    int s = 0;
+    while ((1 << s) < x) ++s;
+    return s;
+}
+
+}  // namespace
+
+static const int GRAPH_STEPS = 8;
+
+struct gmd_ctx {
+    gmd_desc d;
+    std::vector<float> params;
+    std::vector<int> excl;
+    std::vector<float4> h_pos, h_vel;
+    std::vector<int> h_rim;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 0;
+    // device
+    float4 *pos0 = nullptr, *pos1 = nullptr, *vel = nullptr, *frc = nullptr, *rref = nullptr, *obeta = nullptr;
+    float *sc = nullptr;
+    int *rim = nullptr, *nbl = nullptr, *nbc = nullptr;
+    int *cell_of = nullptr, *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_members = nullptr;
+    int *d_excl = nullptr;
+    float *d_params = nullptr;
+    Ctrl *ctrl = nullptr;
+    float *partial = nullptr;
+    int *rebuild_flag = nullptr;
+    int rebuild_cap = 0;
+    float *saver_out = nullptr;
+    size_t nbl_len = 0;
+    // state
+    int buf0 = 0;
+    long long lstep = 0;
+    bool need_rebuild = true;   // forced rebuild pending (first build, gmd_force_rebuild)
+    float last_ms = 0.f;
+    // launch
+    DP P;
+    kfun k_pers = nullptr, k_stp = nullptr;
+    int block = 256, grid_pers = 1, grid_step = 1, grid_aux = 1;
+    size_t lds_pers = 0, lds_step = 0, lds_build = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+struct gmd_rdf {
+    int N, nt, bins, nclass;
+    float4 *pos = nullptr;
+    int *ptype = nullptr;
+    float *d_thr = nullptr;
+    unsigned *part = nullptr;
+    unsigned long long *d_out = nullptr;
+    int grid = 0;
+    float box[3] = {0, 0, 0};
+    std::vector<float> thr;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+int gmd_abi_version(void) { return GMD_ABI_VERSION; }
+
+int gmd_desc_size(void) { return (int)sizeof(gmd_desc); }
+
+const char *gmd_last_error(void) { return g_err.c_str(); }
+
+int gmd_get_device_info(gmd_device_info *out) {
+    if (!out) return fail("gmd_get_device_info: null output");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHECK(hipGetDeviceProperties(&p, dev));
+    out->compute_units = p.multiProcessorCount;
+    out->wavefront_size = p.warpSize;
+    out->max_threads_per_block = p.maxThreadsPerBlock;
+    out->total_mem = (long long)p.totalGlobalMem;
+    snprintf(out->name, sizeof out->name, "%s", p.name);
+    snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    return 0;
+}
+
+static int ctx_free(gmd_ctx *c) {
+    if (!c) return 0;
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    void *ptrs[] = {c->pos0, c->pos1, c->vel, c->frc, c->rref, c->obeta, c->sc, c->rim, c->nbl, c->nbc,
+                    c->cell_of, c->cell_rank, c->cell_count, c->cell_start, c->cell_members, c->d_excl,
+                    c->d_params, c->ctrl, c->partial, c->rebuild_flag, c->saver_out};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
+    if (!desc || !out) return fail("gmd_create: null argument");
+    *out = nullptr;
+    const gmd_desc &d = *desc;
+    if (d.abi_version != GMD_ABI_VERSION)
+        return fail("gmd_create: ABI version %d, library expects %d", d.abi_version, GMD_ABI_VERSION);
+    if (d.D != 3) return fail("gmd_create: only D=3 is supported (got D=%d)", d.D);
+    if (d.N < 1) return fail("gmd_create: N must be positive (got %d)", d.N);
+    if (d.ntypes < 1 || d.ntypes > MAX_TYPES) return fail("gmd_create: ntypes must be in [1,%d]", MAX_TYPES);
+    if (d.pair_base < 0 || d.pair_base > 1) return fail("gmd_create: unknown pair function %d", d.pair_base);
+    if (d.cutoff < 0 || d.cutoff > 2) return fail("gmd_create: unknown cutoff wrapper %d", d.cutoff);
+    if (d.nblist_kind < 0 || d.nblist_kind > 1) return fail("gmd_create: unknown nblist kind %d", d.nblist_kind);
+    if (d.integrator < 0 || d.integrator > 3) return fail("gmd_create: unknown integrator %d", d.integrator);
+    if (d.max_nbs < 1) return fail("gmd_create: max_nbs must be positive");
+    if (!d.params) return fail("gmd_create: params missing");
+    if (!is_pow2(d.tp) || d.tp > 64) return fail("gmd_create: tp must be a power of two <= 64 (got %d)", d.tp);
+    if (d.pb < 1) return fail("gmd_create: pb must be positive");
+    if (d.utilize_niii) return fail("gmd_create: UtilizeNIII=True is not supported by the MI355X path");
+    for (int k = 0; k < 3; ++k)
+        if (!(d.box[k] > 0.f)) return fail("gmd_create: box lengths must be positive");
+    if (d.scalar_save_every > 0 && d.scalar_saves_per_block < 1)
+        return fail("gmd_create: scalar_saves_per_block must be positive");
+
+    gmd_ctx *c = new gmd_ctx();
+    c->d = d;
+    const int N = d.N;
+    c->params.assign(d.params, d.params + (size_t)d.ntypes * d.ntypes * 3);
+    if (d.exclusions && d.exclusions_cols > 0) c->excl.assign(d.exclusions, d.exclusions + (size_t)N * d.exclusions_cols);
+    c->d.params = nullptr;
+    c->d.exclusions = nullptr;
+
+    int rc = 0;
+#define TRY(x)                 \
+    do {                       \
+        if ((rc = (x)) != 0) { \
+            ctx_free(c);       \
+            return rc;         \
+        }                      \
+    } while (0)
+#define HTRY(x)                                                                 \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) {                                                 \
+            ctx_free(c);                                                        \
+            return fail("%s: %s", #x, hipGetErrorString(e_));                   \
+        }                                                                       \
+    } while (0)
+
+    int dev = 0;
+    HTRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HTRY(hipGetDeviceProperties(&prop, dev));
+    c->cus = prop.multiProcessorCount;
+    HTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HTRY(hipEventCreate(&c->ev0));
+    HTRY(hipEventCreate(&c->ev1));
+
+    // ---- cell grid (nblist_linked_lists.py:36-42) ----
+    int ncx = 1, ncy = 1, ncz = 1;
+    if (d.nblist_kind == GMD_NBLIST_LINKED_LISTS) {
+        double min_size = ((double)d.max_cut + (double)d.skin) / 2.0;
+        ncx = (int)((double)d.box[0] / min_size);
+        ncy = (int)((double)d.box[1] / min_size);
+        ncz = (int)((double)d.box[2] / min_size);
+        if (ncx <= 6 || ncy <= 4 || ncz <= 4) {
+            ctx_free(c);
+            return fail("linked lists need more than 6 cells in x and more than 4 in y and z "
+                        "(box/((max_cut+skin)/2) gives %d x %d x %d); use 'N squared'", ncx, ncy, ncz);
+        }
+    }
+    const int ncells = ncx * ncy * ncz;
+
+    // ---- launch geometry ----
+    const int tp = d.tp;
+    int block = d.pb * tp;
+    if (block % 64) block = ((block + 63) / 64) * 64;
+    if (block > 1024) block = 1024;
+    c->block = block;
+    const int slots = block / tp;
+    c->grid_step = (N + slots - 1) / slots;
+    c->grid_aux = std::max(1, std::min((N + block - 1) / block, c->cus * 16));
+    const int wpb = block / 64;
+    int llcap = 512;
+    while (llcap > 64 && (size_t)wpb * llcap * 16 > 64 * 1024) llcap >>= 1;
+    c->lds_build = std::max((size_t)wpb * llcap * 16, (size_t)block * 16);
+    const size_t lds_pp = 0, lds_red = 4096, lds_b = 4096 + 16 * 16 * 8;
+    c->lds_step = lds_b;
+    c->lds_pers = lds_b + c->lds_build;
+
+    // ---- buffers ----
+    const size_t rows = (size_t)(d.max_nbs + tp - 1) / tp;
+    c->nbl_len = rows * (size_t)N * tp;
+    TRY(dalloc(&c->pos0, N));
+    TRY(dalloc(&c->pos1, N));
+    TRY(dalloc(&c->vel, N));
+    TRY(dalloc(&c->frc, N));
+    TRY(dalloc(&c->rref, N));
+    TRY(dalloc(&c->obeta, N));
+    TRY(dalloc(&c->sc, (size_t)5 * N));
+    TRY(dalloc(&c->rim, (size_t)3 * N));
+    TRY(dalloc(&c->nbl, c->nbl_len));
+    TRY(dalloc(&c->nbc, N));
+    TRY(dalloc(&c->cell_of, N));
+    TRY(dalloc(&c->cell_rank, N));
+    TRY(dalloc(&c->cell_count, ncells));
+    TRY(dalloc(&c->cell_start, (size_t)ncells + 1));
+    TRY(dalloc(&c->cell_members, N));
+    TRY(dalloc(&c->d_params, c->params.size()));
+    TRY(dalloc(&c->ctrl, 1));
+    const int max_grid = std::max(c->grid_step, c->cus * 32);
+    TRY(dalloc(&c->partial, (size_t)max_grid * GMD_NRED));
+    TRY(dalloc(&c->saver_out, (size_t)std::max(1, d.scalar_saves_per_block) * 6));
+    if (!c->excl.empty()) {
+        TRY(dalloc(&c->d_excl, c->excl.size()));
+        HTRY(hipMemcpy(c->d_excl, c->excl.data(), c->excl.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HTRY(hipMemcpy(c->d_params, c->params.data(), c->params.size() * sizeof(float), hipMemcpyHostToDevice));
+    HTRY(hipMemset(c->cell_count, 0, (size_t)ncells * sizeof(int)));
+    HTRY(hipMemset(c->sc, 0, (size_t)5 * N * sizeof(float)));
+    HTRY(hipMemset(c->rim, 0, (size_t)3 * N * sizeof(int)));
+    HTRY(hipMemset(c->nbc, 0, (size_t)N * sizeof(int)));
+    HTRY(hipMemset(c->obeta, 0, (size_t)N * sizeof(float4)));
+    HTRY(hipMemset(c->frc, 0, (size_t)N * sizeof(float4)));
+    HTRY(hipMemset(c->rref, 0, (size_t)N * sizeof(float4)));
+    HTRY(hipMemset(c->saver_out, 0, (size_t)std::max(1, d.scalar_saves_per_block) * 6 * sizeof(float)));
+    Ctrl h_ctrl;
+    memset(&h_ctrl, 0, sizeof h_ctrl);
+    h_ctrl.shift_step = -1;
+    HTRY(hipMemcpy(c->ctrl, &h_ctrl, sizeof h_ctrl, hipMemcpyHostToDevice));
+    c->h_pos.assign(N, make_float4(0.f, 0.f, 0.f, 0.f));
+    c->h_vel.assign(N, make_float4(0.f, 0.f, 0.f, 1.f));
+    c->h_rim.assign((size_t)3 * N, 0);
+
+    // ---- device parameter block ----
+    DP &P = c->P;
+    memset(&P, 0, sizeof P);
+    P.N = N;
+    P.ntypes = d.ntypes;
+    P.tp = tp;
+    P.tp_shift = ilog2(tp);
+    P.max_nbs = d.max_nbs;
+    P.nblist_kind = d.nblist_kind;
+    P.niii = 0;
+    P.box0 = d.box[0];
+    P.box1 = d.box[1];
+    P.box2 = d.box[2];
+    {
+        float cps = d.max_cut + d.skin;
+        P.cps2 = cps * cps;
+        P.skin2q = (d.skin * d.skin) * 0.25f;
+    }
+    P.pos0 = c->pos0;
+    P.pos1 = c->pos1;
+    P.vel = c->vel;
+    P.frc = c->frc;
+    P.rref = c->rref;
+    P.obeta = c->obeta;
+    P.sc = c->sc;
+    P.rim = c->rim;
+    P.nbl = c->nbl;
+    P.nbc = c->nbc;
+    P.ncx = ncx;
+    P.ncy = ncy;
+    P.ncz = ncz;
+    P.ncells = ncells;
+    P.llcap = llcap;
+    P.cell_of = c->cell_of;
+    P.cell_rank = c->cell_rank;
+    P.cell_count = c->cell_count;
+    P.cell_start = c->cell_start;
+    P.cell_members = c->cell_members;
+    P.excl = c->d_excl;
+    P.excl_cols = d.exclusions_cols;
+    P.raw_params = c->d_params;
+    P.pair_base = d.pair_base;
+    P.cutoff = d.cutoff;
+    P.dt = d.dt;
+    P.omega2 = d.omega2;
+    P.alpha = d.alpha;
+    P.degrees = (double)d.degrees;
+    P.seed = d.seed;
+    P.tfunc = d.tfunc;
+    P.tpar0 = d.tparams[0];
+    P.tpar1 = d.tparams[1];
+    P.tpar2 = d.tparams[2];
+    P.tpar3 = d.tparams[3];
+    P.compute_u = d.compute_u;
+    P.compute_w = d.compute_w;
+    P.compute_lap = d.compute_lap;
+    P.compute_k = d.compute_k;
+    P.compute_fsq = d.compute_fsq;
+    P.mom_every = d.momentum_reset_every;
+    P.save_every = d.scalar_save_every;
+    P.saves_per_block = d.scalar_saves_per_block;
+    P.ctrl = c->ctrl;
+    P.partial = c->partial;
+    P.saver_out = c->saver_out;
+    P.lds_pp = (int)lds_pp;
+    P.lds_red = (int)lds_red;
+    P.lds_build = (int)lds_b;
+
+    pick_kernels(d.pair_base, d.cutoff, d.integrator, c->k_pers, c->k_stp);
+
+    // ---- cooperative grid: all blocks must be co-resident ----
+    if (d.gridsync) {
+        int nb = 0;
+        HTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)c->k_pers, block, c->lds_pers));
+        if (nb < 1) {
+            ctx_free(c);
+            return fail("gridsync: persistent kernel cannot be resident with %d threads per block", block);
+        }
+        c->grid_pers = std::max(1, std::min(c->grid_step, nb * c->cus));
+    }
+    *out = c;
+    return 0;
+#undef TRY
+#undef HTRY
+}
+
+int gmd_destroy(gmd_ctx *ctx) { return ctx_free(ctx); }
+
+int gmd_upload(gmd_ctx *c, const float *r, const float *v, const float *f, const float *scalars, const int *ptype,
+               const int *r_im) {
+    if (!c) return fail("gmd_upload: null context");
+    const int N = c->d.N;
+    const gmd_desc &d = c->d;
+    if (ptype)
+        for (int i = 0; i < N; ++i) {
+            if (ptype[i] < 0 || ptype[i] >= d.ntypes)
+                return fail("gmd_upload: particle %d has type %d outside [0,%d)", i, ptype[i], d.ntypes);
+            c->h_pos[i].w = i2f(ptype[i]);
+        }
+    if (r)
+        for (int i = 0; i < N; ++i) {
+            c->h_pos[i].x = r[3 * i];
+            c->h_pos[i].y = r[3 * i + 1];
+            c->h_pos[i].z = r[3 * i + 2];
+        }
+    if (v)
+        for (int i = 0; i < N; ++i) {
+            c->h_vel[i].x = v[3 * i];
+            c->h_vel[i].y = v[3 * i + 1];
+            c->h_vel[i].z = v[3 * i + 2];
+        }
+    if (scalars && d.sid_m >= 0)
+        for (int i = 0; i < N; ++i) c->h_vel[i].w = scalars[(size_t)i * d.nscalars + d.sid_m];
+    if (r || ptype) {
+        // the current buffer becomes pos0; whether the list is still valid is
+        // decided by the skin test at the start of the next run
+        c->buf0 = 0;
+        HIPCHECK(hipMemcpy(c->pos0, c->h_pos.data(), (size_t)N * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    if (v || (scalars && d.sid_m >= 0))
+        HIPCHECK(hipMemcpy(c->vel, c->h_vel.data(), (size_t)N * sizeof(float4), hipMemcpyHostToDevice));
+    if (f) {
+        std::vector<float4> h(N);
+        for (int i = 0; i < N; ++i) h[i] = make_float4(f[3 * i], f[3 * i + 1], f[3 * i + 2], 0.f);
+        HIPCHECK(hipMemcpy(c->frc, h.data(), (size_t)N * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    if (scalars) {
+        std::vector<float> h((size_t)5 * N, 0.f);
+        const int sids[5] = {d.sid_u, d.sid_w, d.sid_lap, d.sid_k, d.sid_fsq};
+        for (int col = 0; col < 5; ++col)
+            if (sids[col] >= 0)
+                for (int i = 0; i < N; ++i) h[(size_t)col * N + i] = scalars[(size_t)i * d.nscalars + sids[col]];
+        HIPCHECK(hipMemcpy(c->sc, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (r_im) {
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) c->h_rim[(size_t)k * N + i] = r_im[3 * i + k];
+        HIPCHECK(hipMemcpy(c->rim, c->h_rim.data(), c->h_rim.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int gmd_download(gmd_ctx *c, float *r, float *v, float *f, float *scalars, int *ptype, int *r_im) {
+    if (!c) return fail("gmd_download: null context");
+    const int N = c->d.N;
+    const gmd_desc &d = c->d;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (r || ptype) {
+        HIPCHECK(hipMemcpy(c->h_pos.data(), c->buf0 ? c->pos1 : c->pos0, (size_t)N * sizeof(float4),
+                           hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            if (r) {
+                r[3 * i] = c->h_pos[i].x;
+                r[3 * i + 1] = c->h_pos[i].y;
+                r[3 * i + 2] = c->h_pos[i].z;
+            }
+            if (ptype) ptype[i] = f2i(c->h_pos[i].w);
+        }
+    }
+    if (v || scalars) {
+        HIPCHECK(hipMemcpy(c->h_vel.data(), c->vel, (size_t)N * sizeof(float4), hipMemcpyDeviceToHost));
+        if (v)
+            for (int i = 0; i < N; ++i) {
+                v[3 * i] = c->h_vel[i].x;
+                v[3 * i + 1] = c->h_vel[i].y;
+                v[3 * i + 2] = c->h_vel[i].z;
+            }
+    }
+    if (f) {
+        std::vector<float4> h(N);
+        HIPCHECK(hipMemcpy(h.data(), c->frc, (size_t)N * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            f[3 * i] = h[i].x;
+            f[3 * i + 1] = h[i].y;
+            f[3 * i + 2] = h[i].z;
+        }
+    }
+    if (scalars) {
+        std::vector<float> h((size_t)5 * N);
+        HIPCHECK(hipMemcpy(h.data(), c->sc, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const int sids[5] = {d.sid_u, d.sid_w, d.sid_lap, d.sid_k, d.sid_fsq};
+        for (int col = 0; col < 5; ++col)
+            if (sids[col] >= 0)
+                for (int i = 0; i < N; ++i) scalars[(size_t)i * d.nscalars + sids[col]] = h[(size_t)col * N + i];
+        if (d.sid_m >= 0)
+            for (int i = 0; i < N; ++i) scalars[(size_t)i * d.nscalars + d.sid_m] = c->h_vel[i].w;
+    }
+    if (r_im) {
+        HIPCHECK(hipMemcpy(c->h_rim.data(), c->rim, c->h_rim.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) r_im[3 * i + k] = c->h_rim[(size_t)k * N + i];
+    }
+    return 0;
+}
+
+static int ensure_flags(gmd_ctx *c, int steps) {
+    if (c->rebuild_cap < steps + 2) {
+        if (c->rebuild_flag) HIPCHECK(hipFree(c->rebuild_flag));
+        c->rebuild_flag = nullptr;
+        int cap = std::max(steps + 2, 1024);
+        if (dalloc(&c->rebuild_flag, cap)) return -1;
+        c->rebuild_cap = cap;
+        c->P.rebuild_flag = c->rebuild_flag;
+        if (c->gexec) {
+            hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+        if (c->graph) {
+            hipGraphDestroy(c->graph);
+            c->graph = nullptr;
+        }
+    }
+    return 0;
+}
+
+// One interaction evaluation (+ integration) as a sequence of launches.
+static void enqueue_phase_step(gmd_ctx *c) {
+    const DP &P = c->P;
+    hipStream_t st = c->stream;
+    if (c->d.nblist_kind == GMD_NBLIST_LINKED_LISTS) {
+        const int gc = std::max(1, std::min((P.ncells + 255) / 256, c->cus * 16));
+        hipLaunchKernelGGL(k_ll_assign, dim3(c->grid_aux), dim3(c->block), 0, st, P);
+        hipLaunchKernelGGL(k_ll_scan, dim3(1), dim3(1024), 1024 * sizeof(int), st, P);
+        hipLaunchKernelGGL(k_ll_scatter, dim3(c->grid_aux), dim3(c->block), 0, st, P);
+        hipLaunchKernelGGL(k_ll_sort, dim3(gc), dim3(256), 0, st, P);
+        const int wpb = c->block / 64;
+        const int gb = std::max(1, std::min((P.ncells + wpb - 1) / wpb, c->cus * 8));
+        hipLaunchKernelGGL(k_ll_build, dim3(gb), dim3(c->block), c->lds_build, st, P);
+    } else {
+        const int gb = std::max(1, (P.N + c->block - 1) / c->block);
+        hipLaunchKernelGGL(k_nsq_build, dim3(gb), dim3(c->block), (size_t)c->block * 16, st, P);
+    }
+    hipLaunchKernelGGL(c->k_stp, dim3(c->grid_step), dim3(c->block), c->lds_step, st, P);
+}
+
+static int run_impl(gmd_ctx *c, float t0, int steps) {
+    if (steps < 0) return fail("gmd_run: steps must be non-negative");
+    if (ensure_flags(c, steps)) return -1;
+    hipStream_t st = c->stream;
+    HIPCHECK(hipMemsetAsync(c->rebuild_flag, 0, (size_t)(steps + 2) * sizeof(int), st));
+    if (c->need_rebuild) {
+        int one = 1;
+        HIPCHECK(hipMemcpyAsync(c->rebuild_flag, &one, sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    if (c->d.scalar_save_every > 0)
+        HIPCHECK(hipMemsetAsync(c->saver_out, 0, (size_t)c->d.scalar_saves_per_block * 6 * sizeof(float), st));
+    hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, c->ctrl, steps, t0, c->buf0, c->lstep);
+    hipLaunchKernelGGL(k_skin_check, dim3(c->grid_aux), dim3(c->block), 0, st, c->P);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev0, st));
+    if (c->d.gridsync) {
+        void *args[] = {(void *)&c->P};
+        HIPCHECK(hipLaunchCooperativeKernel((const void *)c->k_pers, dim3(c->grid_pers), dim3(c->block), args,
+                                            (unsigned)c->lds_pers, st));
+    } else if (c->d.use_graph) {
+        if (!c->gexec) {
+            HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < GRAPH_STEPS; ++k) enqueue_phase_step(c);
+            HIPCHECK(hipStreamEndCapture(st, &c->graph));
+            HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        }
+        const int launches = (steps + 1 + GRAPH_STEPS - 1) / GRAPH_STEPS;
+        for (int k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(c->gexec, st));
+    } else {
+        for (int s = 0; s <= steps; ++s) enqueue_phase_step(c);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipEventRecord(c->ev1, st));
+    HIPCHECK(hipEventSynchronize(c->ev1));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    c->buf0 ^= (steps & 1);
+    c->lstep += steps;
+    c->need_rebuild = false;
+    return 0;
+}
+
+int gmd_run(gmd_ctx *c, float time_zero, int steps) {
+    if (!c) return fail("gmd_run: null context");
+    if (c->d.integrator == GMD_INTEGRATOR_NONE && steps > 0)
+        return fail("gmd_run: context has no integrator; only steps=0 (evaluation) is possible");
+    return run_impl(c, time_zero, steps);
+}
+
+int gmd_evaluate(gmd_ctx *c) {
+    if (!c) return fail("gmd_evaluate: null context");
+    return run_impl(c, 0.0f, 0);
+}
+
+int gmd_last_run_ms(gmd_ctx *c, float *ms) {
+    if (!c || !ms) return fail("gmd_last_run_ms: null argument");
+    *ms = c->last_ms;
+    return 0;
+}
+
+int gmd_read_nbflag(gmd_ctx *c, int nbflag[3]) {
+    if (!c || !nbflag) return fail("gmd_read_nbflag: null argument");
+    Ctrl h;
+    HIPCHECK(hipMemcpy(&h, c->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) nbflag[k] = h.nbflag[k];
+    return 0;
+}
+
+int gmd_read_nblist(gmd_ctx *c, int *out) {
+    if (!c || !out) return fail("gmd_read_nblist: null argument");
+    const int N = c->d.N, M = c->d.max_nbs, tp = c->d.tp;
+    std::vector<int> nbl(c->nbl_len), nbc(N);
+    HIPCHECK(hipMemcpy(nbl.data(), c->nbl, nbl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(nbc.data(), c->nbc, (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < N; ++i) {
+        int *row = out + (size_t)i * (M + 1);
+        int n = std::min(nbc[i], M);
+        for (int k = 0; k < n; ++k) row[k] = nbl[((size_t)(k / tp) * N + i) * tp + (k % tp)];
+        for (int k = n; k < M; ++k) row[k] = 0;
+        row[M] = nbc[i];
+    }
+    return 0;
+}
+
+int gmd_force_rebuild(gmd_ctx *c) {
+    if (!c) return fail("gmd_force_rebuild: null context");
+    c->need_rebuild = true;
+    return 0;
+}
+
+int gmd_get_thermostat(gmd_ctx *c, float state[2]) {
+    if (!c || !state) return fail("gmd_get_thermostat: null argument");
+    Ctrl h;
+    HIPCHECK(hipMemcpy(&h, c->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    state[0] = h.xi;
+    state[1] = h.ksum;
+    return 0;
+}
+
+int gmd_set_thermostat(gmd_ctx *c, const float state[2]) {
+    if (!c || !state) return fail("gmd_set_thermostat: null argument");
+    HIPCHECK(hipMemcpy(&c->ctrl->xi, state, 2 * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gmd_read_scalar_output(gmd_ctx *c, float *out) {
+    if (!c || !out) return fail("gmd_read_scalar_output: null argument");
+    if (c->d.scalar_save_every <= 0) return fail("gmd_read_scalar_output: no scalar saver configured");
+    HIPCHECK(hipMemcpy(out, c->saver_out, (size_t)c->d.scalar_saves_per_block * 6 * sizeof(float),
+                       hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// RDF
+// ---------------------------------------------------------------------------
+static int rdf_bin_host(float x, double half, double bw, int bins) {
+    // calculator_radial_distribution.py:101-104 with numba's float64 promotion
+    if (!((double)x < half * half)) return bins;
+    float dd = std::sqrt(x);
+    if (!((double)dd < half)) return bins;
+    int b = (int)std::floor((double)dd / bw);
+    return std::min(b, bins);
+}
+
+static void rdf_thresholds(const float box[3], int bins, std::vector<float> &thr) {
+    float mn = std::min(box[0], std::min(box[1], box[2]));
+    double half = (double)mn / 2.0;
+    double bw = half / bins;
+    thr.assign(bins + 1, 0.f);
+    for (int k = 1; k <= bins; ++k) {
+        // smallest non-negative float x with bin(x) >= k (bin is monotone in x)
+        uint32_t lo = 0, hi = 0x7f800000u;  // +inf
+        while (lo < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            float x;
+            memcpy(&x, &mid, 4);
+            if (rdf_bin_host(x, half, bw, bins) >= k)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        float x;
+        memcpy(&x, &lo, 4);
+        thr[k] = x;
+    }
+}
+
+int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
+    if (!out) return fail("gmd_rdf_create: null output");
+    *out = nullptr;
+    if (N < 1 || ntypes < 1 || ntypes > MAX_TYPES || bins < 1) return fail("gmd_rdf_create: bad sizes");
+    gmd_rdf *r = new gmd_rdf();
+    r->N = N;
+    r->nt = ntypes;
+    r->bins = bins;
+    r->nclass = ntypes * (ntypes + 1) / 2;
+    const size_t nh = (size_t)r->nclass * bins;
+    const size_t lds = ((nh * 4 + (size_t)(bins + 1) * 4 + 15) & ~(size_t)15) + 256 * 16 + 256 * 4;
+    if (lds > 64 * 1024) {
+        delete r;
+        return fail("gmd_rdf_create: %d type pairs x %d bins do not fit the LDS histogram", r->nclass, bins);
+    }
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long T = (N + 255) / 256;
+    const long long npairs = T * (T + 1) / 2;
+    r->grid = (int)std::max(1LL, std::min(npairs, (long long)cus * 8));
+    if (hipMalloc(&r->pos, (size_t)N * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&r->ptype, (size_t)N * sizeof(int)) != hipSuccess ||
+        hipMalloc(&r->d_thr, (size_t)(bins + 1) * sizeof(float)) != hipSuccess ||
+        hipMalloc(&r->part, (size_t)r->grid * nh * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&r->d_out, nh * sizeof(unsigned long long)) != hipSuccess) {
+        gmd_rdf_destroy(r);
+        return fail("gmd_rdf_create: out of device memory");
+    }
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+        gmd_rdf_destroy(r);
+        return fail("gmd_rdf_create: stream creation failed");
+    }
+    *out = r;
+    return 0;
+}
+
+int gmd_rdf_destroy(gmd_rdf *r) {
+    if (!r) return 0;
+    void *ptrs[] = {r->pos, r->ptype, r->d_thr, r->part, r->d_out};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (r->stream) hipStreamDestroy(r->stream);
+    delete r;
+    return 0;
+}
+
+static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3], unsigned long long *counts) {
+    if (box[0] != r->box[0] || box[1] != r->box[1] || box[2] != r->box[2] || r->thr.empty()) {
+        rdf_thresholds(box, r->bins, r->thr);
+        HIPCHECK(hipMemcpy(r->d_thr, r->thr.data(), r->thr.size() * sizeof(float), hipMemcpyHostToDevice));
+        memcpy(r->box, box, sizeof r->box);
+    }
+    RdfP R;
+    R.N = r->N;
+    R.nt = r->nt;
+    R.bins = r->bins;
+    R.nclass = r->nclass;
+    R.tile = 256;
+    R.ntiles = (r->N + 255) / 256;
+    R.npairs = (long long)R.ntiles * (R.ntiles + 1) / 2;
+    R.L0 = box[0];
+    R.L1 = box[1];
+    R.L2 = box[2];
+    {
+        float mn = std::min(box[0], std::min(box[1], box[2]));
+        double bw = ((double)mn / 2.0) / r->bins;
+        R.inv_bw = (float)(1.0 / bw);
+    }
+    R.pos = pos;
+    R.ptype = d_ptype;
+    R.thr2 = r->d_thr;
+    R.part = r->part;
+    const size_t nh = (size_t)r->nclass * r->bins;
+    const size_t lds = ((nh * 4 + (size_t)(r->bins + 1) * 4 + 15) & ~(size_t)15) + 256 * 16 + 256 * 4;
+    hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), lds, r->stream, R);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, r->stream, r->part, r->grid,
+                       (int)nh, r->d_out);
+    HIPCHECK(hipGetLastError());
+    std::vector<unsigned long long> h(nh);
+    HIPCHECK(hipMemcpyAsync(h.data(), r->d_out, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));
+    HIPCHECK(hipStreamSynchronize(r->stream));
+    const int nt = r->nt, bins = r->bins;
+    for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < nt; ++b) {
+            int lo = std::min(a, b), hi = std::max(a, b);
+            int cls = lo * nt - (lo * (lo - 1)) / 2 + (hi - lo);
+            for (int k = 0; k < bins; ++k) {
+                unsigned long long v = h[(size_t)cls * bins + k];
+                counts[((size_t)a * nt + b) * bins + k] = (a == b) ? 2 * v : v;
+            }
+        }
+    return 0;
+}
+
+int gmd_rdf_compute_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, unsigned long long *counts) {
+    if (!r || !c || !counts) return fail("gmd_rdf_compute_ctx: null argument");
+    if (c->d.N != r->N) return fail("gmd_rdf_compute_ctx: N mismatch (%d vs %d)", c->d.N, r->N);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    const int *dp = nullptr;
+    if (ptype) {
+        for (int i = 0; i < r->N; ++i)
+            if (ptype[i] < 0 || ptype[i] >= r->nt) return fail("gmd_rdf_compute_ctx: type out of range");
+        HIPCHECK(hipMemcpy(r->ptype, ptype, (size_t)r->N * sizeof(int), hipMemcpyHostToDevice));
+        dp = r->ptype;
+    } else if (c->d.ntypes > r->nt) {
+        return fail("gmd_rdf_compute_ctx: context has %d types, calculator %d", c->d.ntypes, r->nt);
+    }
+    return rdf_run(r, c->buf0 ? c->pos1 : c->pos0, dp, c->d.box, counts);
+}
+
+int gmd_rdf_compute_host(gmd_rdf *r, const float *pos, const int *ptype, const float box[3],
+                         unsigned long long *counts) {
+    if (!r || !pos || !ptype || !box || !counts) return fail("gmd_rdf_compute_host: null argument");
+    for (int i = 0; i < r->N; ++i)
+        if (ptype[i] < 0 || ptype[i] >= r->nt) return fail("gmd_rdf_compute_host: type out of range");
+    std::vector<float4> h(r->N);
+    for (int i = 0; i < r->N; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], i2f(ptype[i]));
+    HIPCHECK(hipMemcpy(r->pos, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
+    return rdf_run(r, r->pos, nullptr, box, counts);
+}
+
+}  // extern "C"

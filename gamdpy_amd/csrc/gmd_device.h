@@ -1,0 +1,266 @@
+// gmd_device.h -- device-side building blocks shared by the MD kernels.
+//
+// State layout in HBM (one context = one system of N particles):
+//   pos[2]  float4 (x, y, z, type bits)      ping-pong: step s reads pos[buf0^(s&1)]
+//   vel     float4 (vx, vy, vz, mass)
+//   frc     float4 (fx, fy, fz, 0)
+//   sc      float  [5][N]  U, W, lapU, K, Fsq   (column-major, coalesced)
+//   rim     int    [3][N]  image counters (touched only when a particle wraps)
+//   rref    float4 positions at the last neighbour-list rebuild
+//   nbl     int    neighbour list, entry k of particle i at ((k/tp)*N + i)*tp + k%tp
+//   nbc     int    neighbour count (true count; entries beyond max_nbs are dropped)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GMD_NRED 10   // per-block partial sums: K U W lap Fsq px py pz m (unused)
+#define RED_K 0
+#define RED_U 1
+#define RED_W 2
+#define RED_LAP 3
+#define RED_FSQ 4
+#define RED_PX 5
+#define RED_M 8
+
+#define SC_U 0
+#define SC_W 1
+#define SC_LAP 2
+#define SC_K 3
+#define SC_FSQ 4
+
+#define MAX_TYPES 8
+
+// Control block kept in device memory for the lifetime of a context.
+struct Ctrl {
+    float xi, ksum;            // NVT thermostat_state (NVT.py:143)
+    float shift[3];            // pending momentum-reset shift (momentum_reset.py:186)
+    int shift_step;            // step at which the pending shift is applied (-1 = none)
+    int nbflag[3];             // nblist.py:9 [update pending, overflow, #rebuilds]
+    unsigned red_count;        // last-block counter (kernel-per-phase path)
+    unsigned bar_count;        // grid barrier arrival counter
+    unsigned bar_gen;          // grid barrier generation
+    int step;                  // current step (kernel-per-phase path)
+    int nsteps;
+    int buf0;                  // position buffer holding r at step 0 of the run
+    float t0;
+    long long lstep0;          // Langevin counter of step 0 of the run
+    int pad[4];
+};
+
+// Per-type-pair constants, prepared in LDS at kernel start.
+//   LJ sigma/epsilon: a = sigma^2, b = 4 eps, c = 24 eps
+//   LJ 12-6:          a = A12, b = A6, c = 12 A12, d = 6 A6
+struct PairP {
+    float a, b, c, d;
+    float cut2, ucut, scut, cut;
+};
+
+struct DP {
+    int N, ntypes, tp, tp_shift, max_nbs;
+    int nblist_kind, niii;
+    float box0, box1, box2;
+    float cps2;                // fl32(max_cut + skin)^2 (nblist.py:119)
+    float skin2q;              // skin*skin*0.25 (orthorhombic.py:131)
+    float4 *pos0, *pos1, *vel, *frc, *rref, *obeta;
+    float *sc;
+    int *rim;
+    int *nbl, *nbc;
+    // cells
+    int ncx, ncy, ncz, ncells, llcap;
+    int *cell_of, *cell_rank, *cell_count, *cell_start, *cell_members;
+    const int *excl;
+    int excl_cols;
+    const float *raw_params;   // ntypes*ntypes*3
+    int pair_base, cutoff;
+    // integrator
+    float dt, omega2, alpha;
+    double degrees;
+    unsigned long long seed;
+    int tfunc;
+    float tpar0, tpar1, tpar2, tpar3;
+    int compute_u, compute_w, compute_lap, compute_k, compute_fsq;
+    int mom_every, save_every, saves_per_block;
+    Ctrl *ctrl;
+    float *partial;            // [nblocks][GMD_NRED]
+    int *rebuild_flag;         // [nsteps+2]
+    float *saver_out;          // [saves_per_block][6]
+    int lds_pp, lds_red, lds_build;   // byte offsets into dynamic LDS
+};
+
+// ---------------------------------------------------------------------------
+// Orthorhombic box (simulation_boxes/orthorhombic.py:43-132).  The exact
+// float32 operation order matters for neighbour sets and histogram bins, so
+// contraction into FMA is switched off here.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float mic1(float d, float L) {
+#pragma clang fp contract(off)
+    float t = 2.0f * d;
+    return t > L ? d - L : (t < -L ? d + L : d);
+}
+
+__device__ __forceinline__ float dist_sq_mic(float ax, float ay, float az, float bx, float by, float bz,
+                                             float L0, float L1, float L2, float &dx, float &dy, float &dz) {
+#pragma clang fp contract(off)
+    dx = mic1(ax - bx, L0);
+    dy = mic1(ay - by, L1);
+    dz = mic1(az - bz, L2);
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    return d2;
+}
+
+// ---------------------------------------------------------------------------
+// Pair functions in the reference's float32 operation order
+// (LJ_12_6_sigma_epsilon.py:25-36, LJ_12_6.py:34-41), used for the cutoff
+// constants u(rc), s(rc) so they equal the oracle's bit for bit.
+// ---------------------------------------------------------------------------
+__device__ inline void pair_ref(int base, float dist, float p0, float p1, float &u, float &s, float &umm) {
+#pragma clang fp contract(off)
+    if (base == 0) {
+        float od = 1.0f / dist;
+        float x = p0 * od;
+        float x2 = x * x;
+        float x6 = x2 * x2 * x2;
+        float x12 = x6 * x6;
+        float od2 = od * od;
+        u = (4.0f * p1) * (x12 - x6);
+        s = ((24.0f * p1) * (2.0f * x12 - x6)) * od2;
+        umm = ((24.0f * p1) * (26.0f * x12 - 7.0f * x6)) * od2;
+    } else {
+        float inv = 1.0f / dist;
+        float p2 = inv * inv;
+        float p6 = p2 * p2 * p2;
+        float p12 = p6 * p6;
+        float p8 = p6 * p2;
+        float p14 = p12 * p2;
+        u = p0 * p12 + p1 * p6;
+        s = (12.0f * p0) * p14 + (6.0f * p1) * p8;
+        umm = (156.0f * p0) * p14 + (42.0f * p1) * p8;
+    }
+}
+
+__device__ inline void make_pairp(int base, int cutoff, float p0, float p1, float cut, PairP &q) {
+    if (base == 0) {
+        q.a = p0 * p0;
+        q.b = 4.0f * p1;
+        q.c = 24.0f * p1;
+        q.d = 0.0f;
+    } else {
+        q.a = p0;
+        q.b = p1;
+        q.c = 12.0f * p0;
+        q.d = 6.0f * p1;
+    }
+    q.cut = cut;
+    {
+#pragma clang fp contract(off)
+        q.cut2 = cut * cut;
+    }
+    float u, s, umm;
+    pair_ref(base, cut, p0, p1, u, s, umm);
+    q.ucut = (cutoff == 0) ? 0.0f : u;
+    q.scut = (cutoff == 2) ? s : 0.0f;
+}
+
+// Fast evaluation inside the force loop: from d2 directly, one reciprocal.
+template <int BASE, int CUT>
+__device__ __forceinline__ void pair_fast(float d2, const PairP &q, float &u, float &s, float &umm) {
+    float ir2 = __builtin_amdgcn_rcpf(d2);
+    if (BASE == 0) {
+        float x2 = q.a * ir2;
+        float x6 = x2 * x2 * x2;
+        float x12 = x6 * x6;
+        u = q.b * (x12 - x6);
+        s = q.c * (2.0f * x12 - x6) * ir2;
+        umm = q.c * (26.0f * x12 - 7.0f * x6) * ir2;
+    } else {
+        float p6 = ir2 * ir2 * ir2;
+        float p12 = p6 * p6;
+        u = q.a * p12 + q.b * p6;
+        s = (q.c * p12 + q.d * p6) * ir2;
+        umm = (13.0f * q.c * p12 + 7.0f * q.d * p6) * ir2;
+    }
+    if (CUT == 1) {
+        u = u - q.ucut;
+    } else if (CUT == 2) {
+        float r = d2 * __builtin_amdgcn_rsqf(d2);
+        u = u - (q.ucut - q.scut * q.cut * (r - q.cut));
+        s = s - q.scut;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based normal variates: Philox-4x32-10 (Salmon et al., SC'11) and
+// the Box-Muller transform.  counter = (id, step lo, step hi, 0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        if (r < 9) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+    }
+}
+
+__device__ __forceinline__ void normal3(unsigned long long seed, long long step, int id, float &z0, float &z1,
+                                        float &z2) {
+    uint32_t c[4] = {(uint32_t)id, (uint32_t)((unsigned long long)step & 0xFFFFFFFFull),
+                     (uint32_t)((unsigned long long)step >> 32), 0u};
+    philox4x32_10(c, (uint32_t)(seed & 0xFFFFFFFFull), (uint32_t)(seed >> 32));
+    const float inv24 = 5.9604644775390625e-08f;  // 2^-24
+    float u0 = (float)((c[0] >> 8) + 1u) * inv24;
+    float u1 = (float)((c[1] >> 8) + 1u) * inv24;
+    float u2 = (float)((c[2] >> 8) + 1u) * inv24;
+    float u3 = (float)((c[3] >> 8) + 1u) * inv24;
+    const float two_pi = 6.283185307179586f;
+    float r0 = sqrtf(-2.0f * logf(u0));
+    float r1 = sqrtf(-2.0f * logf(u2));
+    float sn, cs;
+    sincosf(two_pi * u1, &sn, &cs);
+    z0 = r0 * cs;
+    z1 = r0 * sn;
+    z2 = r1 * cosf(two_pi * u3);
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Temperature schedules of misc/make_function.py, evaluated in float64 like
+// numba does once the time (float32 + int64*float32) has become float64.
+__device__ inline double target_temperature(const DP &P, double t) {
+    if (P.tfunc == 0) return (double)P.tpar0;
+    if (P.tfunc == 1) {  // ramp(value0, x0, value1, x1)
+        double v0 = P.tpar0, x0 = P.tpar1, v1 = P.tpar2, x1 = P.tpar3;
+        float alpha = (P.tpar2 - P.tpar0) / (P.tpar3 - P.tpar1);
+        if (t < x0) return v0;
+        if (t < x1) return v0 + (t - x0) * (double)alpha;
+        return v1;
+    }
+    // sin(period, amplitude, offset)
+    double period = P.tpar0, amp = P.tpar1, off = P.tpar2;
+    return off + amp * sin(2.0 * 3.141592653589793 * t / period);
+}

@@ -1,0 +1,61 @@
+"""Evaluator: interactions only, on a given configuration (tools/Evaluator.py:10-95)."""
+import numpy as np
+
+from . import _native
+from .configuration import get_default_compute_flags
+from .interactions import PairPotential
+
+
+class Evaluator:
+    """Evaluate forces, energies and virials of ``configuration`` once.
+
+    Parameters
+    ----------
+    configuration : Configuration
+    interactions : a PairPotential (or a one-element list)
+    compute_plan : optional compute plan
+    """
+
+    def __init__(self, configuration, interactions, compute_plan=None, verbose=True):
+        from .simulation import get_default_compute_plan
+        self.configuration = configuration
+        self.compute_plan = get_default_compute_plan(configuration) if compute_plan is None else compute_plan
+        self.interactions = interactions if type(interactions) == list else [interactions, ]
+        if len(self.interactions) != 1 or not isinstance(self.interactions[0], PairPotential):
+            raise NotImplementedError("the MI355X Evaluator takes exactly one PairPotential")
+        flags = get_default_compute_flags()
+        for k in ('lapU', 'Fsq'):
+            flags[k] = configuration.compute_flags[k]
+        conf = configuration
+        pp = self.interactions[0]
+        ip = pp.get_params(conf, self.compute_plan)
+        table = ip["table"]
+        fields = {
+            "N": conf.N, "D": conf.D, "ntypes": table.shape[0], "box": conf.simbox.lengths,
+            "pair_base": _native.PAIR_BASE[ip["pair_base"]], "cutoff": _native.CUTOFF[ip["cutoff"]],
+            "max_cut": float(ip["max_cut"]), "nblist_kind": _native.NBLIST[ip["nblist_kind"]],
+            "skin": float(ip["skin"]), "max_nbs": int(ip["max_nbs"]),
+            "utilize_niii": int(bool(self.compute_plan.get('UtilizeNIII', False))),
+            "integrator": 0, "dt": 0.0,
+            "compute_u": int(flags['U']), "compute_w": int(flags['W']), "compute_lap": int(flags['lapU']),
+            "compute_k": 0, "compute_fsq": 0,
+            "nscalars": len(conf.scalar_columns),
+            "sid_u": conf.sid.get('U', -1), "sid_w": conf.sid.get('W', -1), "sid_lap": conf.sid.get('lapU', -1),
+            "sid_k": conf.sid.get('K', -1), "sid_fsq": conf.sid.get('Fsq', -1), "sid_m": conf.sid['m'],
+            "pb": int(self.compute_plan['pb']), "tp": int(self.compute_plan['tp']),
+            "gridsync": 0, "use_graph": 0,
+        }
+        self.ctx = _native.Context(fields, table, ip["exclusions"])
+        pp.nblist.bind(self.ctx)
+
+    def evaluate(self, configuration=None):
+        """Upload, evaluate interactions, download (Evaluator.py:82-95)."""
+        if configuration is None:
+            configuration = self.configuration
+        owner = configuration._ctx
+        configuration.attach(self.ctx)
+        configuration.copy_to_device()
+        self.ctx.evaluate()
+        configuration.copy_to_host()
+        configuration.attach(owner)
+        return

@@ -1,0 +1,200 @@
+"""PairPotential and the neighbour lists (interactions/pair_potential.py,
+interactions/nblist.py, interactions/nblist_linked_lists.py).
+
+The reference's ``get_params``/``get_kernel`` plugin protocol returns numba
+device arrays and device functions.  Here ``get_params`` returns the same
+information as a plain descriptor that the Simulation packs into the C ABI's
+``gmd_desc``; the kernels themselves are the HIP templates of libgamdpy_amd.
+"""
+import numpy as np
+
+from .potential_functions import pair_function_descriptor
+
+
+class Interaction:
+    """Abstract interaction protocol (interactions/interaction.py:7-33)."""
+
+    def get_kernel(self, configuration, compute_plan, compute_flags):
+        raise NotImplementedError
+
+    def get_params(self, configuration, compute_plan):
+        raise NotImplementedError
+
+    def check_datastructure_validity(self) -> bool:
+        return True
+
+
+class _DeviceArrayView:
+    """Stand-in for a numba device array: ``copy_to_host()`` reads from the context."""
+
+    def __init__(self, fetch):
+        self._fetch = fetch
+
+    def copy_to_host(self):
+        return self._fetch()
+
+
+class _NbListBase:
+    kind = None
+
+    def __init__(self, configuration, exclusions, max_num_nbs):
+        self.configuration = configuration
+        self.N, self.D = configuration.N, configuration.D
+        self.max_nbs = int(max_num_nbs)
+        self.exclusions = exclusions
+        self.nbflag = np.zeros(3, dtype=np.int32)
+        self._ctx = None
+
+    def _exclusion_array(self):
+        if type(self.exclusions) == np.ndarray:
+            ex = np.ascontiguousarray(self.exclusions, dtype=np.int32)
+            if ex.ndim != 2 or ex.shape[0] != self.N or ex.shape[1] < 1:
+                raise ValueError("exclusions must have shape (N, max_exclusions+1) with the count in the last column")
+            if np.any(ex[:, -1] > ex.shape[1] - 1) or np.any(ex[:, -1] < 0):
+                raise ValueError("exclusion counts (last column) exceed the row length")
+            return ex
+        return None
+
+    def get_params(self, max_cut, compute_plan, verbose=False):
+        self.max_cut = np.float32(max_cut)
+        self.skin, = [compute_plan[key] for key in ['skin']]
+        return {"nblist_kind": self.kind, "skin": np.float32(self.skin), "max_nbs": self.max_nbs,
+                "max_cut": self.max_cut, "exclusions": self._exclusion_array()}
+
+    def bind(self, ctx):
+        self._ctx = ctx
+
+    @property
+    def d_nbflag(self):
+        if self._ctx is None:
+            return _DeviceArrayView(lambda: self.nbflag.copy())
+        return _DeviceArrayView(self._ctx.nbflag)
+
+    @property
+    def d_nblist(self):
+        if self._ctx is None:
+            raise RuntimeError("neighbour list has no device state yet (create a Simulation or Evaluator first)")
+        return _DeviceArrayView(self._ctx.nblist)
+
+
+class NbList2(_NbListBase):
+    """O(N^2) neighbour list (interactions/nblist.py:5-163)."""
+    kind = "N squared"
+
+
+class NbListLinkedLists(_NbListBase):
+    """Cell-linked-list neighbour list (interactions/nblist_linked_lists.py:7-300)."""
+    kind = "linked lists"
+
+    def get_params(self, max_cut, compute_plan, verbose=False):
+        out = super().get_params(max_cut, compute_plan, verbose)
+        # nblist_linked_lists.py:36-42
+        min_cells_size = (self.max_cut + self.skin) / 2
+        self.cells_per_dimension = np.zeros(self.D, dtype=np.int32)
+        for i in range(self.D):
+            self.cells_per_dimension[i] = int(self.configuration.simbox.lengths[i] / min_cells_size)
+            assert self.cells_per_dimension[i] > 4
+            if i == 0:
+                assert self.cells_per_dimension[i] > 6
+        return out
+
+
+class PairPotential(Interaction):
+    """Pair potential (interactions/pair_potential.py:9-268).
+
+    Parameters
+    ----------
+    pairpotential_function : a supported pair function (see potential_functions)
+    params : list of scalars or (ntypes, ntypes) arrays; the last one is the cutoff
+    max_num_nbs : neighbour-list capacity per particle
+    exclusions : optional (N, k+1) int array, count in the last column
+    """
+
+    def __init__(self, pairpotential_function, params, max_num_nbs, exclusions=None):
+        self.pairpotential_function = pairpotential_function
+        self.params_user = params
+        self.exclusions = exclusions
+        self.max_num_nbs = max_num_nbs
+        self.base, self.cutoff = pair_function_descriptor(pairpotential_function)
+        self.nblist = None
+
+    def convert_user_params(self):
+        """pair_potential.py:23-51: (ntypes, ntypes) record array of (p0, p1, ..., cut) and max_cut."""
+        params_list = []
+        for parameter in self.params_user:
+            if np.isscalar(parameter):
+                params_list.append(np.ones((1, 1)) * parameter)
+            else:
+                params_list.append(np.array(parameter, dtype=np.float32))
+        num_types = params_list[0].shape[0]
+        for parameter in params_list:
+            assert len(parameter.shape) == 2
+            assert parameter.shape[0] == num_types
+            assert parameter.shape[1] == num_types
+        params = np.zeros((num_types, num_types), dtype="f," * len(params_list))
+        for i in range(num_types):
+            for j in range(num_types):
+                params[i, j] = tuple(p[i, j] for p in params_list)
+        max_cut = np.float32(np.max(params_list[-1]))
+        return params, max_cut
+
+    def param_table(self):
+        """(ntypes, ntypes, 3) float32 table (p0, p1, cut) as the kernels read it."""
+        params, max_cut = self.convert_user_params()
+        nt = params.shape[0]
+        names = params.dtype.names
+        table = np.zeros((nt, nt, 3), np.float32)
+        table[:, :, 0] = params[names[0]]
+        table[:, :, 1] = params[names[1]] if len(names) > 2 else 0.0
+        table[:, :, 2] = params[names[-1]]
+        return table, max_cut
+
+    def evaluate_potential_function(self, r, types):
+        params, max_cut = self.convert_user_params()
+        u, s, lap = self.pairpotential_function(r, params[types[0], types[1]])
+        return u
+
+    def plot(self, xlim=None, ylim=(-3, 6), figsize=(8, 4), names=None):
+        import matplotlib.pyplot as plt
+        params, max_cut = self.convert_user_params()
+        num_types = len(params[0])
+        if names is None:
+            names = np.arange(num_types)
+        plt.figure(figsize=figsize)
+        for i in range(num_types):
+            for j in range(num_types):
+                r = np.linspace(0.5, params[i, j][-1], 1000)
+                u, s, lap = self.pairpotential_function(r, params[i, j])
+                plt.plot(r, u, label=f'{names[i]} - {names[j]}')
+        if xlim is not None:
+            plt.xlim(xlim)
+        plt.ylim(ylim)
+        plt.xlabel('Pair distance')
+        plt.ylabel('Pair potential')
+        plt.legend()
+        plt.show()
+
+    def check_datastructure_validity(self) -> bool:
+        """pair_potential.py:78-82"""
+        nbflag = self.nblist.d_nbflag.copy_to_host()
+        if nbflag[0] != 0 or nbflag[1] != 0:
+            raise RuntimeError(f'Neighbor-list is invalid. Try allocating space for more neighbors (max_num_nbs in '
+                               f'PairPot object). Allocated size: {self.max_num_nbs}, but {nbflag[1] + 1} neighbours '
+                               f'found. {nbflag=}.')
+        return True
+
+    def get_params(self, configuration, compute_plan, verbose=False):
+        """pair_potential.py:84-97: parameter table and a fresh neighbour list."""
+        table, max_cut = self.param_table()
+        if compute_plan['nblist'] == 'N squared':
+            self.nblist = NbList2(configuration, self.exclusions, self.max_num_nbs)
+        elif compute_plan['nblist'] == 'linked lists':
+            self.nblist = NbListLinkedLists(configuration, self.exclusions, self.max_num_nbs)
+        else:
+            raise ValueError(f"No lblist called: {compute_plan['nblist']}. Use either 'N squared' or 'linked lists'")
+        nblist_params = self.nblist.get_params(max_cut, compute_plan, verbose)
+        return {"pair_base": self.base, "cutoff": self.cutoff, "table": table, **nblist_params}
+
+    def get_kernel(self, configuration, compute_plan, compute_flags, verbose=False):
+        """The device code is fixed HIP; this returns its descriptor."""
+        return {"pair_base": self.base, "cutoff": self.cutoff}

@@ -1,0 +1,99 @@
+"""Runtime actions compiled into the step kernel (runtime_actions/ of the reference).
+
+MomentumReset and ScalarSaver are executed inside the fused HIP step:
+their per-particle sums are reduced per block and finished by the last block
+of the step (deterministic order), instead of the reference's float atomics.
+"""
+import numpy as np
+
+
+class RuntimeAction:
+    """runtime_actions/runtime_action.py:7-56"""
+
+    def get_compute_flags(self):
+        return None
+
+    def setup(self, configuration, num_timeblocks: int, steps_per_timeblock: int, output, verbose=False) -> None:
+        pass
+
+    def get_params(self, configuration, compute_plan) -> dict:
+        return {}
+
+    def get_prestep_kernel(self, configuration, compute_plan):
+        return None
+
+    def get_poststep_kernel(self, configuration, compute_plan):
+        return None
+
+    def update_at_end_of_timeblock(self, timeblock: int, output_reference, ctx=None):
+        pass
+
+    def initialize_before_timeblock(self):
+        pass
+
+
+class MomentumReset(RuntimeAction):
+    """Zero the total momentum every ``steps_between_reset`` steps of a timeblock
+    (momentum_reset.py:8-103)."""
+
+    def __init__(self, steps_between_reset: int) -> None:
+        if type(steps_between_reset) != int or steps_between_reset < 0:
+            raise ValueError(f'steps_between_momentum_reset ({steps_between_reset}) should be non-negative integer.')
+        self.steps_between_reset = steps_between_reset
+
+    def get_params(self, configuration, compute_plan):
+        return {"momentum_reset_every": self.steps_between_reset}
+
+
+class ScalarSaver(RuntimeAction):
+    """Sum of per-particle U, W, [lapU], K, [Fsq] (and Vol) every
+    ``steps_between_output`` steps (scalar_saver.py:8-206)."""
+
+    def __init__(self, steps_between_output: int = 16, compute_flags=None, verbose=False) -> None:
+        if type(steps_between_output) != int or steps_between_output < 0:
+            raise ValueError(f'steps_between_output ({steps_between_output}) should be non-negative integer.')
+        self.steps_between_output = steps_between_output
+        self.compute_flags = compute_flags
+
+    def get_compute_flags(self):
+        return self.compute_flags
+
+    def setup(self, configuration, num_timeblocks: int, steps_per_timeblock: int, output, verbose=False) -> None:
+        self.configuration = configuration
+        if type(num_timeblocks) != int or num_timeblocks < 0:
+            raise ValueError(f'num_timeblocks ({num_timeblocks}) should be non-negative integer.')
+        self.num_timeblocks = num_timeblocks
+        if type(steps_per_timeblock) != int or steps_per_timeblock < 0:
+            raise ValueError(f'steps_per_timeblock ({steps_per_timeblock}) should be non-negative integer.')
+        self.steps_per_timeblock = steps_per_timeblock
+        if self.steps_between_output >= steps_per_timeblock:
+            raise ValueError(f'scalar_output ({self.steps_between_output}) must be less than steps_per_timeblock '
+                             f'({steps_per_timeblock})')
+        compute_flags = configuration.compute_flags
+        if compute_flags['Ptot'] or compute_flags['stresses']:
+            raise NotImplementedError("ScalarSaver: Ptot and stresses columns are not supported by the MI355X path")
+        self.num_scalars = 0
+        self.sid = {}
+        for item in ['U', 'W', 'lapU', 'K', 'Fsq', 'Vol']:
+            if compute_flags[item]:
+                self.sid[item] = self.num_scalars
+                self.num_scalars += 1
+        self.scalar_saves_per_block = self.steps_per_timeblock // self.steps_between_output
+        shape = (self.num_timeblocks, self.scalar_saves_per_block, self.num_scalars)
+        if 'scalars' in output.keys():
+            del output['scalars']
+        output.create_dataset('scalars', shape=shape, dtype=np.float32)
+        output.attrs['steps_between_output'] = self.steps_between_output
+        output.attrs['scalar_names'] = list(self.sid.keys())
+        self.output_array = np.zeros((self.scalar_saves_per_block, self.num_scalars), np.float32)
+
+    def get_params(self, configuration, compute_plan):
+        return {"scalar_save_every": self.steps_between_output,
+                "scalar_saves_per_block": self.scalar_saves_per_block}
+
+    def update_at_end_of_timeblock(self, block: int, output, ctx=None):
+        from ._native import SAVE_COLUMNS
+        raw = ctx.scalar_output()
+        cols = [SAVE_COLUMNS.index(name) for name in self.sid]
+        self.output_array = raw[:, cols]
+        output['scalars'][block, :] = self.output_array

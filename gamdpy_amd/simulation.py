@@ -1,0 +1,321 @@
+"""Simulation: timeblocks of the fused MD step on the GPU (simulation/Simulation.py).
+
+Same constructor, ``run``, ``run_timeblocks``, ``status``, ``summary``,
+``integrate_self`` and ``autotune`` as the reference.  Instead of composing
+numba device functions into a kernel, the constructor packs the interaction,
+integrator, runtime actions and compute plan into one ``gmd_desc`` and creates
+a native context (libgamdpy_amd.so) that owns the device state.
+"""
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import _native
+from .configuration import get_default_compute_flags
+from .interactions import PairPotential
+from .output import open_output
+
+
+def _ll_cells_ok(configuration, max_cut, skin):
+    """nblist_linked_lists.py:36-42 admissibility."""
+    size = (float(max_cut) + float(skin)) / 2
+    n = [int(float(L) / size) for L in configuration.simbox.lengths]
+    return all(c > 4 for c in n) and n[0] > 6
+
+
+def get_default_compute_plan(configuration):
+    """Default compute plan for MI355X (gfx950: 256 CUs, 64-wide wavefronts).
+
+    Same keys as simulation/get_default_compute_plan.py:8-89:
+      pb   particles per block,   tp  lanes (threads) per particle,
+      skin neighbour-list skin,   UtilizeNIII  (half lists; not used on MI355X),
+      gridsync  one persistent cooperative launch per timeblock,
+      nblist    'N squared' or 'linked lists'.
+    The lane count fills the chip with at least 16 wavefronts per CU; the
+    persistent launch is used while the grid stays co-resident.
+    """
+    N = configuration.N
+    try:
+        cus = _native.device_info()["compute_units"]
+    except Exception:
+        cus = 256
+    tp = 1
+    while N * tp < cus * 64 * 16 and tp < 16:
+        tp *= 2
+    pb = max(1, 256 // tp)
+    skin = 0.5
+    gridsync = N * tp <= cus * 64 * 32
+    nblist = 'N squared' if N <= 6000 else 'linked lists'
+    return {'pb': pb, 'tp': tp, 'skin': skin, 'UtilizeNIII': False, 'gridsync': gridsync, 'nblist': nblist}
+
+
+class Simulation:
+    """Run a configuration forward in time with the given interactions, integrator
+    and runtime actions (simulation/Simulation.py:15-180)."""
+
+    def __init__(self, configuration, interactions, integrator, runtime_actions,
+                 num_steps=0, num_timeblocks=0, steps_per_timeblock=0,
+                 compute_plan=None, storage='output.h5', verbose=False, timing=True,
+                 steps_in_kernel_test=1):
+        self.configuration = configuration
+        if compute_plan is None:
+            self.compute_plan = get_default_compute_plan(self.configuration)
+        else:
+            self.compute_plan = compute_plan
+        if type(interactions) == list:
+            self.interactions = interactions
+        else:
+            self.interactions = [interactions, ]
+        if len(self.interactions) != 1 or not isinstance(self.interactions[0], PairPotential):
+            raise NotImplementedError("the MI355X path runs exactly one PairPotential per Simulation")
+        self.integrator = integrator
+        self.dt = self.integrator.dt
+
+        if num_timeblocks == 0:
+            if num_steps == 0:
+                raise ValueError("Either num_steps or num_timeblocks must be non-zero")
+            num_timeblocks = 32
+            steps_per_timeblock = 2 ** int(math.log2(math.ceil(num_steps / num_timeblocks)))
+            num_timeblocks = math.ceil(num_steps / steps_per_timeblock)
+            print('num_steps: ', num_steps)
+            print('num_blocks: ', num_timeblocks)
+            print('steps_per_block: ', steps_per_timeblock)
+        elif steps_per_timeblock == 0:
+            raise ValueError("If num_timeblocks is non-zero then steps_per_timeblock must be too "
+                             "(num_steps is ignored in this case)")
+        self.num_blocks = num_timeblocks
+        self.current_block = -1
+        self.steps_per_block = steps_per_timeblock
+        self.storage = storage
+        self.timing = timing
+        self.steps_in_kernel_test = steps_in_kernel_test
+        self.verbose = verbose
+
+        self.memory = open_output(storage)
+        self.memory.attrs['dt'] = self.dt
+        self.memory.attrs['simbox_initial'] = self.configuration.simbox.lengths
+        if 'ptype' in self.memory.keys():
+            del self.memory['ptype']
+        self.memory.create_dataset("ptype", data=configuration.ptype, shape=(self.configuration.N,), dtype=np.int32)
+        self.memory.attrs['script_name'] = sys.argv[0] if sys.argv else ''
+
+        self.runtime_actions = runtime_actions
+        compute_flags = None
+        for runtime_action in self.runtime_actions:
+            if runtime_action.get_compute_flags() is not None:
+                if compute_flags is not None:
+                    raise ValueError('Can not handle more than one compute_flags in runtime_actions')
+                compute_flags = runtime_action.get_compute_flags()
+        self.compute_flags = get_default_compute_flags()
+        if compute_flags is not None:
+            for k in compute_flags:
+                if k in self.compute_flags:
+                    self.compute_flags[k] = compute_flags[k]
+                else:
+                    raise ValueError('Unknown key in compute_flags:%s' % k)
+        for k in self.compute_flags:
+            if self.compute_flags[k] and not configuration.compute_flags[k]:
+                raise ValueError('compute_flags["%s]" set for Simulation but not in Configuration' % k)
+        for runtime_action in self.runtime_actions:
+            runtime_action.setup(configuration=self.configuration, num_timeblocks=num_timeblocks,
+                                 steps_per_timeblock=steps_per_timeblock, output=self.memory)
+        self.ctx = None
+        self.timing_numba_blocks = np.array([])
+        self.JIT_and_test_kernel()
+        for interaction in self.interactions:
+            interaction.check_datastructure_validity()
+
+    # ------------------------------------------------------------------
+    @property
+    def output(self):
+        return self.memory
+
+    def get_output(self, mode="r"):
+        return self.memory
+
+    def _descriptor(self):
+        conf = self.configuration
+        plan = self.compute_plan
+        pp = self.interactions[0]
+        ip = pp.get_params(conf, plan)
+        table = ip["table"]
+        ntypes = table.shape[0]
+        if conf.N and (np.max(conf.ptype) >= ntypes or np.min(conf.ptype) < 0):
+            raise ValueError(f"particle types must lie in [0, {ntypes}) for this PairPotential")
+        fields = {
+            "N": conf.N, "D": conf.D, "ntypes": ntypes, "box": conf.simbox.lengths,
+            "pair_base": _native.PAIR_BASE[ip["pair_base"]], "cutoff": _native.CUTOFF[ip["cutoff"]],
+            "max_cut": float(ip["max_cut"]),
+            "nblist_kind": _native.NBLIST[ip["nblist_kind"]], "skin": float(ip["skin"]),
+            "max_nbs": int(ip["max_nbs"]), "utilize_niii": int(bool(plan.get('UtilizeNIII', False))),
+            "compute_u": int(self.compute_flags['U']), "compute_w": int(self.compute_flags['W']),
+            "compute_lap": int(self.compute_flags['lapU']), "compute_k": int(self.compute_flags['K']),
+            "compute_fsq": int(self.compute_flags['Fsq']),
+            "nscalars": len(conf.scalar_columns),
+            "sid_u": conf.sid.get('U', -1), "sid_w": conf.sid.get('W', -1), "sid_lap": conf.sid.get('lapU', -1),
+            "sid_k": conf.sid.get('K', -1), "sid_fsq": conf.sid.get('Fsq', -1), "sid_m": conf.sid['m'],
+            "pb": int(plan['pb']), "tp": int(plan['tp']), "gridsync": int(bool(plan['gridsync'])),
+            "use_graph": int(bool(plan.get('graph', True))),
+        }
+        fields.update(self.integrator.get_params(conf, ip))
+        seen = set()
+        for ra in self.runtime_actions:
+            for k, v in ra.get_params(conf, plan).items():
+                if k in seen:
+                    raise NotImplementedError(f"more than one runtime action sets {k}")
+                seen.add(k)
+                fields[k] = v
+        if self.compute_flags['Ptot'] or self.compute_flags['stresses']:
+            raise NotImplementedError("Ptot/stresses are not supported by the MI355X path")
+        return fields, table, ip["exclusions"]
+
+    def _make_context(self):
+        if self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
+        fields, table, excl = self._descriptor()
+        self.ctx = _native.Context(fields, table, excl)
+        self.configuration.attach(self.ctx)
+        self.interactions[0].nblist.bind(self.ctx)
+
+    def JIT_and_test_kernel(self, adjust_compute_plan=True):
+        """Create the native context and run steps_in_kernel_test steps on a copy of
+        the configuration (Simulation.py:200-224).  A plan the device cannot run is
+        stepped down: tp is halved, then gridsync is switched off."""
+        while True:
+            try:
+                self._make_context()
+                self.configuration.copy_to_device()
+                self.integrator.upload_state(self.ctx)
+                self.integrate_self(0.0, self.steps_in_kernel_test)
+                self.integrator.download_state(self.ctx)
+                break
+            except _native.NativeError as e:
+                msg = str(e)
+                if not ("resident" in msg or "cooperative" in msg.lower() or "out of resources" in msg.lower()):
+                    raise
+                if not adjust_compute_plan:
+                    self.compute_plan['tp'] = 0
+                    break
+                if self.compute_plan['tp'] > 1:
+                    self.compute_plan['tp'] //= 2
+                elif self.compute_plan['gridsync']:
+                    self.compute_plan['gridsync'] = False
+                else:
+                    raise
+                print('Trying adjusted compute_plan :', self.compute_plan)
+
+    def integrate_self(self, time_zero, steps):
+        """Simulation.py:271-282: steps integration steps, steps+1 interaction evaluations."""
+        self.ctx.run(np.float32(time_zero), steps)
+
+    def run(self, verbose=True):
+        for _ in self.run_timeblocks():
+            if verbose:
+                print(self.status(per_particle=True))
+        if verbose:
+            print(self.summary())
+
+    def run_timeblocks(self, num_timeblocks=-1):
+        """Generator running one timeblock per iteration (Simulation.py:367-451)."""
+        if num_timeblocks == -1:
+            num_timeblocks = self.num_blocks
+        self.last_num_blocks = num_timeblocks
+        assert num_timeblocks <= self.num_blocks
+        self.configuration.attach(self.ctx)
+        self.configuration.copy_to_device()
+        self.integrator.upload_state(self.ctx)
+        self.vectors_list = []
+        self.scalars_list = []
+        self.simbox_data_list = []
+        self.scalars_t = []
+        block_times = []
+        t_start = time.perf_counter()
+        for block in range(num_timeblocks):
+            self.current_block = block
+            for runtime_action in self.runtime_actions:
+                runtime_action.initialize_before_timeblock()
+            self.integrate_self(np.float32(block * self.steps_per_block * self.dt), self.steps_per_block)
+            if self.timing:
+                block_times.append(self.ctx.last_run_ms())
+            self.configuration.copy_to_host()
+            self.integrator.download_state(self.ctx)
+            for interaction in self.interactions:
+                interaction.check_datastructure_validity()
+            for runtime_action in self.runtime_actions:
+                runtime_action.update_at_end_of_timeblock(block, self.output, ctx=self.ctx)
+            yield block
+        if self.timing:
+            self.timing_numba = (time.perf_counter() - t_start) * 1000.0
+            self.timing_numba_blocks = np.array(block_times)
+        self.scalars_list = np.array(self.scalars_list)
+
+    def status(self, per_particle=False) -> str:
+        time_ = (self.current_block + 1) * self.steps_per_block * self.dt
+        st = f'timeblock= {self.current_block :<6}'
+        st += f'{time_= :<12.3f}'.replace('time_', 'time')
+        for name in self.configuration.sid:
+            if name in self.configuration.compute_flags and self.configuration.compute_flags[name]:
+                data = np.sum(self.configuration[name])
+                if per_particle:
+                    data /= self.configuration.N
+                st += f'{name}= {data:<10.3f}'
+        return st
+
+    def summary(self) -> str:
+        st = f'Particles : {self.configuration.N} \n'
+        st += f'Steps : {self.last_num_blocks} * {self.steps_per_block} = '
+        st += f'{self.last_num_blocks * self.steps_per_block:_} \n'
+        if self.timing:
+            time_total = self.timing_numba / 1000
+            tps_total = self.last_num_blocks * self.steps_per_block / time_total
+            time_sim = np.sum(self.timing_numba_blocks) / 1000
+            tps_sim = self.last_num_blocks * self.steps_per_block / time_sim
+            st += f'Total run time  (incl. time spent between timeblocks): {time_total:.2f} s '
+            st += f'( TPS: {tps_total:.2e} )\n'
+            st += f'Simulation time (excl. time spent between timeblocks): {time_sim:.2f} s '
+            st += f'( TPS: {tps_sim:.2e} )\n'
+        return st
+
+    # ------------------------------------------------------------------
+    def autotune(self, tps=None, skins=None, timesteps=0, repeats=1, verbose=False):
+        """Search lanes-per-particle, driver (persistent / graph) and skin for the
+        fastest plan (Simulation.py:636-821 does the same over pb, tp, skin,
+        gridsync and nblist).  Leaves the fastest plan in self.compute_plan."""
+        if timesteps == 0:
+            timesteps = self.steps_per_block
+        base = dict(self.compute_plan)
+        tps = tps or [1, 2, 4, 8, 16]
+        skins = skins or [base['skin'] - 0.2, base['skin'], base['skin'] + 0.2]
+        best = (float('inf'), dict(base))
+        for gridsync in ([True, False] if base['gridsync'] else [False]):
+            for tp in tps:
+                for skin in skins:
+                    if skin <= 0:
+                        continue
+                    plan = dict(base, tp=tp, pb=max(1, 256 // tp), gridsync=gridsync, skin=float(skin))
+                    if plan['nblist'] == 'linked lists' and not _ll_cells_ok(
+                            self.configuration, self.interactions[0].param_table()[1], skin):
+                        continue
+                    self.compute_plan = plan
+                    try:
+                        self._make_context()
+                        self.configuration.copy_to_device()
+                        self.integrator.upload_state(self.ctx)
+                        self.integrate_self(0.0, 2)
+                        t = 0.0
+                        for _ in range(repeats):
+                            self.configuration.copy_to_device()
+                            self.integrate_self(0.0, timesteps)
+                            t += self.ctx.last_run_ms()
+                    except _native.NativeError:
+                        continue
+                    if verbose:
+                        print(f'{plan} : {repeats * timesteps / t * 1000:.1f} TPS')
+                    if t < best[0]:
+                        best = (t, plan)
+        self.compute_plan = best[1]
+        self.JIT_and_test_kernel()
+        return self.compute_plan

@@ -1,0 +1,157 @@
+/*
+ * gamdpy_amd.h -- C ABI of the MI355X (gfx950) per-timestep MD library.
+ *
+ * The reference (gamdpy) composes numba.cuda device functions into one fused
+ * kernel per Simulation; this library replaces that kernel and its launch with
+ * hand-written HIP kernels behind the entry points below.  Every signature uses
+ * plain C types and host pointers; device memory is owned by the context.
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative value on failure;
+ *     gmd_last_error() then holds a message for the calling thread;
+ *   - host arrays are C-contiguous and use the reference's layouts
+ *     (positions (N,3) float32, scalars (N,nscalars) float32, ...);
+ *   - calls are synchronous on return; one context per host thread.
+ */
+#ifndef GAMDPY_AMD_H
+#define GAMDPY_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMD_ABI_VERSION 3
+
+typedef struct gmd_ctx gmd_ctx;
+typedef struct gmd_rdf gmd_rdf;
+typedef struct gmd_batch gmd_batch;
+
+/* pair function (interactions/potential_functions/) */
+enum { GMD_LJ_12_6_SIGMA_EPSILON = 0, GMD_LJ_12_6 = 1 };
+/* cutoff wrapper: none, apply_shifted_potential_cutoff, apply_shifted_force_cutoff */
+enum { GMD_CUT_PLAIN = 0, GMD_CUT_SHIFTED_POTENTIAL = 1, GMD_CUT_SHIFTED_FORCE = 2 };
+/* compute_plan['nblist'] */
+enum { GMD_NBLIST_N_SQUARED = 0, GMD_NBLIST_LINKED_LISTS = 1 };
+/* integrators/; NONE = interactions only (tools/Evaluator.py) */
+enum { GMD_INTEGRATOR_NONE = 0, GMD_NVE = 1, GMD_NVT = 2, GMD_NVT_LANGEVIN = 3 };
+/* misc/make_function.py: constant(value), ramp(value0,x0,value1,x1), sin(period,amplitude,offset) */
+enum { GMD_T_CONSTANT = 0, GMD_T_RAMP = 1, GMD_T_SIN = 2 };
+/* scalar-saver output columns (fixed order; the host maps them to scalar_names) */
+enum { GMD_SAVE_U = 0, GMD_SAVE_W = 1, GMD_SAVE_LAPU = 2, GMD_SAVE_K = 3, GMD_SAVE_FSQ = 4, GMD_SAVE_VOL = 5,
+       GMD_SAVE_NCOLS = 6 };
+
+/*
+ * Everything Simulation.__init__ (simulation/Simulation.py:73-180) gathers from
+ * the Configuration, the PairPotential + NbList, the integrator, the runtime
+ * actions and the compute plan.
+ */
+typedef struct gmd_desc {
+    int abi_version;          /* must equal GMD_ABI_VERSION */
+    int N, D, ntypes;         /* D must be 3 (the linked-list cell index is 3-D in the reference) */
+    float box[3];             /* Orthorhombic lengths */
+
+    /* PairPotential (interactions/pair_potential.py:12-51) */
+    int pair_base, cutoff;
+    const float *params;      /* ntypes*ntypes*3 float32: (p0, p1, cut) for [ti][tj] */
+    float max_cut;            /* max over the cut matrix (pair_potential.py:49) */
+
+    /* NbList2 / NbListLinkedLists (interactions/nblist.py, nblist_linked_lists.py) */
+    int nblist_kind;
+    float skin;
+    int max_nbs;
+    int utilize_niii;         /* compute_plan['UtilizeNIII'] */
+    const int *exclusions;    /* N x exclusions_cols, last column = count; NULL = none */
+    int exclusions_cols;
+
+    /* integrator */
+    int integrator;
+    float dt;
+    float omega2;             /* NVT.py:148 */
+    long long degrees;        /* NVT.py:149 */
+    float alpha;              /* NVT_Langevin.py */
+    unsigned long long seed;  /* NVT_Langevin.py */
+    int tfunc;                /* temperature schedule kind */
+    float tparams[4];
+
+    /* compute flags (simulation/get_default_compute_flags.py) */
+    int compute_u, compute_w, compute_lap, compute_k, compute_fsq;
+
+    /* column map of the reference scalars array (Configuration.py:109-131) */
+    int nscalars, sid_u, sid_w, sid_lap, sid_k, sid_fsq, sid_m;   /* -1 = absent */
+
+    /* runtime actions (runtime_actions/momentum_reset.py, scalar_saver.py) */
+    int momentum_reset_every;      /* 0 = off */
+    int scalar_save_every;         /* 0 = off */
+    int scalar_saves_per_block;
+
+    /* compute plan (simulation/get_default_compute_plan.py) */
+    int pb, tp, gridsync;
+    int use_graph;                 /* kernel-per-phase path: replay a captured hipGraph */
+} gmd_desc;
+
+typedef struct gmd_device_info {
+    int compute_units;
+    int wavefront_size;
+    int max_threads_per_block;
+    long long total_mem;
+    char name[64];
+    char arch[64];
+} gmd_device_info;
+
+/* Library / device ------------------------------------------------------- */
+int gmd_abi_version(void);
+/* sizeof(gmd_desc) as compiled into the library (binding layout check) */
+int gmd_desc_size(void);
+const char *gmd_last_error(void);
+/* replaces the numba.cuda device queries of get_default_compute_plan.py:24-44 */
+int gmd_get_device_info(gmd_device_info *out);
+
+/* Context: replaces Simulation.get_kernels_and_params + make_integrator
+ * (Simulation.py:226-347) and the device arrays owned by Configuration,
+ * NbList and the integrators. */
+int gmd_create(const gmd_desc *desc, gmd_ctx **out);
+int gmd_destroy(gmd_ctx *ctx);
+
+/* Configuration.copy_to_device / copy_to_host (Configuration.py:227-243):
+ * r, v, f (N,3); scalars (N,nscalars); ptype (N,); r_im (N,3).  NULL = skip. */
+int gmd_upload(gmd_ctx *ctx, const float *r, const float *v, const float *f, const float *scalars,
+               const int *ptype, const int *r_im);
+int gmd_download(gmd_ctx *ctx, float *r, float *v, float *f, float *scalars, int *ptype, int *r_im);
+
+/* Simulation.integrate_self (Simulation.py:271-282): 'steps' integration steps
+ * and steps+1 interaction evaluations, starting at time time_zero. */
+int gmd_run(gmd_ctx *ctx, float time_zero, int steps);
+/* Evaluator.evaluate (tools/Evaluator.py:82-95) on the uploaded state */
+int gmd_evaluate(gmd_ctx *ctx);
+/* GPU milliseconds of the last gmd_run (hipEvents around the launch) */
+int gmd_last_run_ms(gmd_ctx *ctx, float *ms);
+
+/* NbList.d_nbflag (nblist.py:9): [update pending, overflow count, #rebuilds] */
+int gmd_read_nbflag(gmd_ctx *ctx, int nbflag[3]);
+/* NbList.d_nblist in the reference layout (N, max_nbs+1), count in the last column */
+int gmd_read_nblist(gmd_ctx *ctx, int *out);
+/* force a rebuild before the next interaction evaluation (nblist.py:67-69 force_update) */
+int gmd_force_rebuild(gmd_ctx *ctx);
+
+/* NVT.thermostat_state (NVT.py:143): [xi, sum K] */
+int gmd_get_thermostat(gmd_ctx *ctx, float state[2]);
+int gmd_set_thermostat(gmd_ctx *ctx, const float state[2]);
+
+/* ScalarSaver.d_output_array (scalar_saver.py:89-94) of the last block:
+ * (scalar_saves_per_block, GMD_SAVE_NCOLS) float32 */
+int gmd_read_scalar_output(gmd_ctx *ctx, float *out);
+
+/* Radial distribution (calculators/calculator_radial_distribution.py:76-119).
+ * Counts of ordered pairs (ntypes, ntypes, bins) as uint64. */
+int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out);
+int gmd_rdf_destroy(gmd_rdf *rdf);
+/* positions from the context's current device state (no host copy) */
+int gmd_rdf_compute_ctx(gmd_rdf *rdf, gmd_ctx *ctx, const int *ptype, unsigned long long *counts);
+/* positions from a host array (N,3) */
+int gmd_rdf_compute_host(gmd_rdf *rdf, const float *r, const int *ptype, const float box[3],
+                         unsigned long long *counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GAMDPY_AMD_H */

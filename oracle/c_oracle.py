@@ -1,0 +1,81 @@
+"""ctypes binding of oracle/cpu_md.c -- test infrastructure only."""
+import ctypes
+
+import numpy as np
+
+from .build_c_oracle import library_path
+
+_lib = None
+_f = ctypes.POINTER(ctypes.c_float)
+_i = ctypes.POINTER(ctypes.c_int)
+_ll = ctypes.POINTER(ctypes.c_longlong)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(library_path())
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_rdf_counts.argtypes = [ctypes.c_int, _f, _i, _f, ctypes.c_int, ctypes.c_int, _ll]
+        L.orc_build_ll.argtypes = [ctypes.c_int, _f, _f, ctypes.c_float, ctypes.c_float, ctypes.c_int, _i, _i]
+        L.orc_forces_sp.argtypes = [ctypes.c_int, _f, _i, _f, ctypes.c_int, _f, _i, _i, ctypes.c_int, _f, _f, _f]
+        L.orc_md_nvt_ll.restype = ctypes.c_longlong
+        L.orc_md_nvt_ll.argtypes = [ctypes.c_int, _f, _f, _f, _i, _i, _f, ctypes.c_int, _f, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_longlong, _f, ctypes.c_int, _f, _f, _f, _f]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def rdf_counts(r, ptype, box, bins, ntypes):
+    r = np.ascontiguousarray(r, np.float32)
+    pt = np.ascontiguousarray(ptype, np.int32)
+    bx = np.ascontiguousarray(box, np.float32)
+    out = np.zeros((ntypes, ntypes, bins), np.int64)
+    lib().orc_rdf_counts(len(r), _p(r, _f), _p(pt, _i), _p(bx, _f), ntypes, bins, _p(out, _ll))
+    return out
+
+
+def build_ll(r, box, max_cut, skin, max_nbs):
+    r = np.ascontiguousarray(r, np.float32)
+    bx = np.ascontiguousarray(box, np.float32)
+    N = len(r)
+    lst = np.zeros((N, max_nbs), np.int32)
+    cnt = np.zeros(N, np.int32)
+    mx = lib().orc_build_ll(N, _p(r, _f), _p(bx, _f), float(max_cut), float(skin), max_nbs, _p(lst, _i), _p(cnt, _i))
+    if mx < 0:
+        raise ValueError("too few cells")
+    return [np.sort(lst[i, :min(cnt[i], max_nbs)]) for i in range(N)], cnt
+
+
+def md_nvt_ll(r, v, m, ptype, box, table, max_cut, skin, max_nbs, dt, T, tau, steps, xi=0.0):
+    """Run 'steps' NVT steps in place-copies; returns dict of final state."""
+    r = np.array(r, np.float32)
+    v = np.array(v, np.float32)
+    m = np.ascontiguousarray(m, np.float32)
+    pt = np.ascontiguousarray(ptype, np.int32)
+    img = np.zeros(r.shape, np.int32)
+    bx = np.ascontiguousarray(box, np.float32)
+    tb = np.ascontiguousarray(table, np.float32)
+    N = len(r)
+    f = np.zeros_like(r)
+    U = np.zeros(N, np.float32)
+    W = np.zeros(N, np.float32)
+    K = np.zeros(N, np.float32)
+    xs = np.array([xi, 0.0], np.float32)
+    omega2 = np.float32(4.0 * np.pi * np.pi / tau / tau)
+    deg = 3 * N - 3
+    nreb = lib().orc_md_nvt_ll(N, _p(r, _f), _p(v, _f), _p(m, _f), _p(pt, _i), _p(img, _i), _p(bx, _f), tb.shape[0],
+                               _p(tb, _f), float(max_cut), float(skin), int(max_nbs), float(dt), float(T),
+                               float(omega2), deg, _p(xs, _f), int(steps), _p(f, _f), _p(U, _f), _p(W, _f), _p(K, _f))
+    if nreb < 0:
+        raise RuntimeError("C oracle: too few cells or neighbour overflow")
+    return {"r": r, "v": v, "f": f, "U": U, "W": W, "K": K, "xi": xs[0], "rebuilds": nreb, "img": img}

@@ -1,0 +1,689 @@
+"""CPU oracle for gamdpy's per-timestep MD path.  TEST INFRASTRUCTURE ONLY.
+
+Only the test-suite, the smoke check and the ``cpu_baseline`` leg of the
+benchmark import this module.  It is the checker the HIP path is compared
+against, never part of the product path.
+
+Each function restates, in NumPy, the algorithm of the reference file:line
+named in its docstring.  Float32 work is written element-wise in the same
+operation order as the reference kernels (NumPy never fuses a multiply and an
+add, so every product and sum is rounded to float32 exactly once).  Where the
+reference promotes to float64 (numba follows NumPy's scalar promotion rules:
+a float32 combined with an int64 or a float64 literal becomes float64) the
+oracle does the same.
+
+What pins each function is listed in ``tests/test_oracle.py``:
+known-answer tests carried over from the reference's own tests, the rumd3
+golden snapshots shipped with the reference (independent C++/CUDA code),
+closed forms and brute-force float64 evaluations.
+"""
+import math
+
+import numpy as np
+
+F32 = np.float32
+F64 = np.float64
+
+
+# ----------------------------------------------------------------------------
+# Simulation box: simulation_boxes/orthorhombic.py:43-132
+# ----------------------------------------------------------------------------
+def mic_dr(ri, rj, box):
+    """dr = ri - rj with the single-image correction of orthorhombic.py:46-55.
+
+    ``dr += -L if 2dr > L else (+L if 2dr < -L else 0)`` per component, f32.
+    """
+    ri = np.asarray(ri, F32)
+    rj = np.asarray(rj, F32)
+    box = np.asarray(box, F32)
+    dr = ri - rj
+    two_dr = F32(2.0) * dr
+    return np.where(two_dr > box, dr - box, np.where(two_dr < -box, dr + box, dr)).astype(F32)
+
+
+def dist_sq_from_dr(dr):
+    """Sum of squares in component order k=0..D-1 (orthorhombic.py:54), f32."""
+    dr = np.asarray(dr, F32)
+    d2 = np.zeros(dr.shape[:-1], F32)
+    for k in range(dr.shape[-1]):
+        d2 = d2 + dr[..., k] * dr[..., k]
+    return d2
+
+
+def dist_sq(ri, rj, box):
+    """orthorhombic.py:59-73 (get_dist_sq_function)."""
+    return dist_sq_from_dr(mic_dr(ri, rj, box))
+
+
+def apply_pbc(r, image, box):
+    """orthorhombic.py:75-86: one wrap per call, image counter updated."""
+    r = np.array(r, F32, copy=True)
+    image = np.array(image, np.int32, copy=True)
+    box = np.asarray(box, F32)
+    for k in range(r.shape[-1]):
+        up = r[..., k] * F32(2.0) > box[k]
+        r[..., k] = np.where(up, r[..., k] - box[k], r[..., k])
+        image[..., k] += up.astype(np.int32)
+        dn = r[..., k] * F32(2.0) < -box[k]
+        r[..., k] = np.where(dn, r[..., k] + box[k], r[..., k])
+        image[..., k] -= dn.astype(np.int32)
+    return r, image
+
+
+def moved_beyond_half_skin(r, r_ref, box, skin):
+    """orthorhombic.py:120-131: dist_sq(r, r_ref) > skin*skin*0.25 (f32)."""
+    skin = F32(skin)
+    return dist_sq(r, r_ref, box) > (skin * skin) * F32(0.25)
+
+
+# ----------------------------------------------------------------------------
+# Lattice and velocities: configuration/make_lattice.py:1-60,
+# configuration/Configuration.py:260-281,373-385
+# ----------------------------------------------------------------------------
+FCC = {"fractional_coordinates": [[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]],
+       "lattice_constants": [1.0, 1.0, 1.0]}
+
+
+def make_lattice(unit_cell, cells, rho=None):
+    """make_lattice.py:26-60: replicate, scale to rho, centre on [-L/2, L/2) (f64)."""
+    pos = np.asarray(unit_cell["fractional_coordinates"], F64)
+    lat = np.asarray(unit_cell["lattice_constants"], F64)
+    cells = list(cells)
+    n_cells = int(np.prod(cells))
+    idx = np.array(np.unravel_index(np.arange(n_cells), cells), F64).T  # (n_cells, D)
+    positions = (idx[:, None, :] + pos[None, :, :]).reshape(-1, pos.shape[1]) * lat
+    box = lat * np.asarray(cells, F64)
+    if rho is not None:
+        vol_pp = np.prod(box) / len(positions)
+        scale = ((1.0 / rho) / vol_pp) ** (1.0 / 3.0)
+        positions = positions * scale
+        box = box * scale
+    positions = positions - box / 2.0
+    return positions, box
+
+
+def random_velocities(N, D, T, seed, m=1.0):
+    """Configuration.py:373-385 then set_kinetic_temperature (:260-267).
+
+    NumPy legacy RandomState, one column at a time, centre-of-mass drift
+    removed per column, rescaled to exactly T with ndof = D(N-1).
+    """
+    v = np.zeros((N, D), F32)
+    np.random.seed(seed)
+    m_arr = np.broadcast_to(np.asarray(m, F32), (N,))
+    for k in range(D):
+        v[:, k] = np.random.normal(v[:, k], (T / m_arr) ** 0.5)
+        drift = np.mean(m_arr * v[:, k]) / np.mean(m_arr)
+        v[:, k] -= drift
+    m_vec = m_arr
+    ndofs = D * (N - 1)
+    T_ = np.sum(np.dot(m_vec, np.sum(v ** 2, axis=1))) / ndofs
+    v *= (T / T_) ** 0.5
+    return v
+
+
+# ----------------------------------------------------------------------------
+# Pair functions: potential_functions/LJ_12_6_sigma_epsilon.py:25-36,
+# LJ_12_6.py:34-41, apply_shifted_potential_cutoff.py:41-47,
+# apply_shifted_force_cutoff.py:29-37
+# ----------------------------------------------------------------------------
+def _pow6_12(x):
+    x2 = x * x
+    x6 = x2 * x2 * x2
+    return x6, x6 * x6
+
+
+def lj_sigma_epsilon(dist, sigma, epsilon):
+    """LJ_12_6_sigma_epsilon.py:25-36 in f32: returns (u, s=-u'/r, u'')."""
+    dist = np.asarray(dist, F32)
+    sigma = np.asarray(sigma, F32)
+    epsilon = np.asarray(epsilon, F32)
+    od = F32(1.0) / dist
+    x6, x12 = _pow6_12(sigma * od)
+    od2 = od * od
+    u = (F32(4.0) * epsilon) * (x12 - x6)
+    s = ((F32(24.0) * epsilon) * (F32(2.0) * x12 - x6)) * od2
+    umm = ((F32(24.0) * epsilon) * (F32(26.0) * x12 - F32(7.0) * x6)) * od2
+    return u, s, umm
+
+
+def lj_12_6(dist, a12, a6):
+    """LJ_12_6.py:34-41 in f32."""
+    dist = np.asarray(dist, F32)
+    a12 = np.asarray(a12, F32)
+    a6 = np.asarray(a6, F32)
+    inv = F32(1.0) / dist
+    p2 = inv * inv
+    p6 = p2 * p2 * p2
+    p12 = p6 * p6
+    p8 = p6 * p2
+    p14 = p12 * p2
+    u = a12 * p12 + a6 * p6
+    s = (F32(12.0) * a12) * p14 + (F32(6.0) * a6) * p8
+    umm = (F32(156.0) * a12) * p14 + (F32(42.0) * a6) * p8
+    return u, s, umm
+
+
+BASE_FUNCTIONS = {"LJ_12_6_sigma_epsilon": lj_sigma_epsilon, "LJ_12_6": lj_12_6}
+
+
+def pair_eval(base, cutoff, dist, p0, p1, cut):
+    """Pair function with the cutoff wrapper applied (f32).
+
+    cutoff 'SP': apply_shifted_potential_cutoff.py:41-47, u -= u(rc).
+    cutoff 'SF': apply_shifted_force_cutoff.py:29-37,
+                 u -= u(rc) - s(rc)*rc*(r - rc);  s -= s(rc).
+    cutoff None: the bare function.
+    """
+    f = BASE_FUNCTIONS[base]
+    u, s, umm = f(dist, p0, p1)
+    if cutoff is None:
+        return u, s, umm
+    cut = np.asarray(cut, F32)
+    u_c, s_c, _ = f(cut, p0, p1)
+    if cutoff == "SP":
+        return u - u_c, s, umm
+    if cutoff == "SF":
+        u = u - (u_c - (s_c * cut) * (np.asarray(dist, F32) - cut))
+        return u, s - s_c, umm
+    raise ValueError(cutoff)
+
+
+# ----------------------------------------------------------------------------
+# Neighbour lists: interactions/nblist.py:89-143 (N squared),
+# interactions/nblist_linked_lists.py:36-42,117-193 (cell lists)
+# ----------------------------------------------------------------------------
+def list_cutoff_sq(max_cut, skin):
+    """nblist.py:119 / nblist_linked_lists.py:173: (max_cut+skin)**2 in f32."""
+    cps = F32(max_cut) + F32(skin)
+    return cps * cps
+
+
+def _niii_keep(i, j, N):
+    """nblist.py:112-116: 2(j-i) in (0, N] or 2(j-i) < -N."""
+    two_dn = 2 * (j - i)
+    return ((two_dn > 0) & (two_dn <= N)) | (two_dn < -N)
+
+
+def _excluded(i, j, exclusions):
+    if exclusions is None:
+        return np.zeros(np.shape(j), bool)
+    cnt = exclusions[i, -1]
+    row = exclusions[i, :cnt]
+    return np.isin(j, row)
+
+
+def nblist_nsquared(r, box, max_cut, skin, niii=False, exclusions=None):
+    """nblist.py:89-143: every j != i (or the NIII half) with
+    dist_sq(r_j, r_i) < (max_cut+skin)^2.  Returns a list of sorted int arrays."""
+    r = np.asarray(r, F32)
+    N = r.shape[0]
+    thr = list_cutoff_sq(max_cut, skin)
+    out = []
+    j_all = np.arange(N)
+    for i in range(N):
+        d2 = dist_sq(r, r[i], box)
+        keep = d2 < thr
+        if niii:
+            keep &= _niii_keep(i, j_all, N)
+        else:
+            keep[i] = False
+        js = j_all[keep]
+        if exclusions is not None:
+            js = js[~_excluded(i, js, exclusions)]
+        out.append(js.astype(np.int32))
+    return out
+
+
+def cells_per_dimension(box, max_cut, skin):
+    """nblist_linked_lists.py:36-42 (with the same size assertions)."""
+    min_size = (float(F32(max_cut)) + float(skin)) / 2
+    n = [int(float(F32(L)) / min_size) for L in box]
+    assert all(c > 4 for c in n) and n[0] > 6, "too few cells for linked lists"
+    return np.array(n, np.int64)
+
+
+def cell_index(r, box, n_cells):
+    """nblist_linked_lists.py:127: floor((r + 0.5*L)*n/L) mod n, evaluated in f64."""
+    r64 = np.asarray(r, F32).astype(F64)
+    L64 = np.asarray(box, F32).astype(F64)
+    c = np.floor((r64 + 0.5 * L64) * n_cells / L64).astype(np.int64)
+    return np.mod(c, n_cells)
+
+
+def nblist_linked_lists(r, box, max_cut, skin, niii=False, exclusions=None):
+    """nblist_linked_lists.py:117-193: bin into cells, walk the 5x5x5 stencil."""
+    r = np.asarray(r, F32)
+    N = r.shape[0]
+    n = cells_per_dimension(box, max_cut, skin)
+    cell = cell_index(r, box, n)
+    flat = (cell[:, 0] * n[1] + cell[:, 1]) * n[2] + cell[:, 2]
+    order = np.argsort(flat, kind="stable")
+    starts = np.searchsorted(flat[order], np.arange(n.prod() + 1))
+    thr = list_cutoff_sq(max_cut, skin)
+    out = []
+    for i in range(N):
+        cands = []
+        for ix in range(-2, 3):
+            for iy in range(-2, 3):
+                for iz in range(-2, 3):
+                    c = (((cell[i, 0] + ix) % n[0]) * n[1] + (cell[i, 1] + iy) % n[1]) * n[2] + (cell[i, 2] + iz) % n[2]
+                    cands.append(order[starts[c]:starts[c + 1]])
+        js = np.concatenate(cands)
+        if niii:
+            js = js[_niii_keep(i, js, N)]
+        else:
+            js = js[js != i]
+        d2 = dist_sq(r[js], r[i], box)
+        js = js[d2 < thr]
+        if exclusions is not None:
+            js = js[~_excluded(i, js, exclusions)]
+        out.append(np.sort(js).astype(np.int32))
+    return out
+
+
+def nblist_to_array(rows, max_nbs):
+    """Reference layout nblist.py:8: (N, max_nbs+1) int32, count in the last column."""
+    N = len(rows)
+    arr = np.zeros((N, max_nbs + 1), np.int32)
+    for i, row in enumerate(rows):
+        c = min(len(row), max_nbs)
+        arr[i, :c] = row[:c]
+        arr[i, -1] = len(row)
+    return arr
+
+
+# ----------------------------------------------------------------------------
+# Forces: interactions/pair_potential.py:141-246 (plain full-list calculator)
+# ----------------------------------------------------------------------------
+def pair_forces(r, ptype, box, params, rows, base="LJ_12_6_sigma_epsilon", cutoff="SP", niii=False):
+    """calc_forces (pair_potential.py:185-246) with the full-list calculator
+    (:162-178) or the NIII calculator (:141-160), accumulated in list order.
+
+    params: (nt, nt, 3) array of (p0, p1, cut).  Returns f (N,3), U, W, lapU (f32).
+    """
+    r = np.asarray(r, F32)
+    ptype = np.asarray(ptype, np.int64)
+    N, D = r.shape
+    params = np.asarray(params, F32)
+    lens = np.array([len(x) for x in rows])
+    maxlen = int(lens.max()) if N else 0
+    J = np.zeros((N, max(maxlen, 1)), np.int64)
+    for i, row in enumerate(rows):
+        J[i, :len(row)] = row
+    f = np.zeros((N, D), F32)
+    U = np.zeros(N, F32)
+    W = np.zeros(N, F32)
+    lap = np.zeros(N, F32)
+    half = F32(0.5)
+    vf = F32(1.0 / D) if niii else F32(0.5 / D)
+    one_m_d = F32(1 - D)
+    ti = ptype
+    idx = np.arange(N)
+    for m in range(maxlen):
+        act = idx[lens > m]
+        j = J[act, m]
+        dr = mic_dr(r[j], r[act], box)  # other - self
+        d2 = dist_sq_from_dr(dr)
+        pp = params[ti[act], ptype[j]]
+        cut = pp[:, 2]
+        inside = d2 < cut * cut
+        a = act[inside]
+        if len(a) == 0:
+            continue
+        jj = j[inside]
+        drr = dr[inside]
+        dist = np.sqrt(d2[inside])
+        u, s, umm = pair_eval(base, cutoff, dist, pp[inside, 0], pp[inside, 1], cut[inside])
+        for k in range(D):
+            fk = drr[:, k] * s
+            f[a, k] = f[a, k] - fk
+            if niii:
+                np.add.at(f[:, k], jj, fk)
+            W[a] = W[a] + ((drr[:, k] * drr[:, k]) * s) * vf
+        if niii:
+            U[a] = U[a] + u
+            lap[a] = lap[a] + (one_m_d * s + umm) * F32(2.0)
+        else:
+            U[a] = U[a] + half * u
+            lap[a] = lap[a] + (one_m_d * s + umm)
+    return f, U, W, lap
+
+
+def pair_forces_f64(r, ptype, box, params, base="LJ_12_6_sigma_epsilon", cutoff="SP"):
+    """Brute-force all-pairs float64 evaluation (no list): the exact answer the
+    float32 path approximates.  Returns f, U, W (per particle, full-list split)."""
+    r = np.asarray(r, F64)
+    box = np.asarray(box, F64)
+    ptype = np.asarray(ptype, np.int64)
+    params = np.asarray(params, F64)
+    N, D = r.shape
+    f = np.zeros((N, D))
+    U = np.zeros(N)
+    W = np.zeros(N)
+    for i in range(N):
+        dr = r - r[i]
+        dr -= box * np.rint(dr / box)
+        d2 = np.sum(dr * dr, axis=1)
+        pp = params[ptype[i], ptype]
+        cut = pp[:, 2]
+        m = (d2 < cut * cut)
+        m[i] = False
+        d = np.sqrt(d2[m])
+        p0, p1, rc = pp[m, 0], pp[m, 1], cut[m]
+
+        def base_f(x):
+            if base == "LJ_12_6_sigma_epsilon":
+                sr6 = (p0 / x) ** 6
+                return 4 * p1 * (sr6 * sr6 - sr6), 24 * p1 * (2 * sr6 * sr6 - sr6) / x ** 2
+            ir6 = x ** -6.0
+            return p0 * ir6 * ir6 + p1 * ir6, (12 * p0 * ir6 * ir6 + 6 * p1 * ir6) / x ** 2
+
+        u, s = base_f(d)
+        uc, sc = base_f(rc)
+        if cutoff == "SP":
+            u = u - uc
+        elif cutoff == "SF":
+            u = u - uc + sc * rc * (d - rc)
+            s = s - sc
+        f[i] = -np.sum(dr[m] * s[:, None], axis=0)
+        U[i] = 0.5 * np.sum(u)
+        W[i] = np.sum(d * d * s) * (0.5 / D)
+    return f, U, W
+
+
+# ----------------------------------------------------------------------------
+# Integrators: integrators/NVE.py:59-105, NVT.py:33-37,75-124,
+# NVT_Langevin.py:88-131
+# ----------------------------------------------------------------------------
+def nve_step(r, v, f, m, image, box, dt):
+    """NVE.py:59-105 (leapfrog, K from the mean half-step velocity), f32."""
+    r = np.array(r, F32, copy=True)
+    v = np.array(v, F32, copy=True)
+    f = np.asarray(f, F32)
+    m = np.asarray(m, F32)
+    dt = F32(dt)
+    D = r.shape[1]
+    K = np.zeros(len(m), F32)
+    fsq = np.zeros(len(m), F32)
+    for k in range(D):
+        fsq = fsq + f[:, k] * f[:, k]
+        v_mean = v[:, k].copy()
+        v[:, k] = v[:, k] + (f[:, k] / m) * dt
+        v_mean = v_mean + v[:, k]
+        v_mean = v_mean / F32(2.0)
+        K = K + ((F32(0.5) * m) * v_mean) * v_mean
+        r[:, k] = r[:, k] + v[:, k] * dt
+    r, image = apply_pbc(r, image, box)
+    return r, v, K, fsq, image
+
+
+def nvt_step(r, v, f, m, image, box, dt, xi):
+    """NVT.py:188-224: v = plus*(minus*v + f/m*dt); K at the new half step."""
+    r = np.array(r, F32, copy=True)
+    v = np.array(v, F32, copy=True)
+    f = np.asarray(f, F32)
+    m = np.asarray(m, F32)
+    dt = F32(dt)
+    factor = (F32(0.5) * F32(xi)) * dt
+    plus = F32(1.0) / (F32(1.0) + factor)
+    minus = F32(1.0) - factor
+    D = r.shape[1]
+    K = np.zeros(len(m), F32)
+    fsq = np.zeros(len(m), F32)
+    for k in range(D):
+        fsq = fsq + f[:, k] * f[:, k]
+        v[:, k] = plus * (minus * v[:, k] + (f[:, k] / m) * dt)
+        K = K + ((F32(0.5) * m) * v[:, k]) * v[:, k]
+        r[:, k] = r[:, k] + v[:, k] * dt
+    r, image = apply_pbc(r, image, box)
+    return r, v, K, fsq, image
+
+
+def nvt_thermostat_update(xi, k_sum, dt, omega2, degrees, target_T):
+    """NVT.py:226-237 with numba's promotions: the denominator degrees*T is
+    int64*float32 -> float64, so the deviation and the increment are float64
+    and the result is stored back into the float32 state."""
+    ke_dev = F64(F32(2.0) * F32(k_sum)) / (F64(degrees) * F64(target_T)) - 1.0
+    inc = F64(F32(dt) * F32(omega2)) * ke_dev
+    return F32(F64(F32(xi)) + inc)
+
+
+def nvt_omega2(tau):
+    """NVT.py:148."""
+    return F32(4.0 * np.pi * np.pi / tau / tau)
+
+
+def langevin_step(r, v, f, m, image, box, dt, alpha, temperature, normals, old_beta):
+    """NVT_Langevin.py:342-385 (Gronbech-Jensen/Farago leapfrog), f32.
+
+    normals: (N, D) standard normal variates for this step."""
+    r = np.array(r, F32, copy=True)
+    v = np.array(v, F32, copy=True)
+    old_beta = np.array(old_beta, F32, copy=True)
+    f = np.asarray(f, F32)
+    m = np.asarray(m, F32)
+    dt = F32(dt)
+    alpha = F32(alpha)
+    T = F32(temperature)
+    D = r.shape[1]
+    K = np.zeros(len(m), F32)
+    fsq = np.zeros(len(m), F32)
+    amp = np.sqrt(((F32(2.0) * alpha) * T) * dt)
+    for k in range(D):
+        beta = amp * np.asarray(normals[:, k], F32)
+        num = F32(2.0) * m - alpha * dt
+        den = F32(2.0) * m + alpha * dt
+        a = num / den
+        bom = F32(2.0) / den
+        K = K + ((F32(0.5) * m) * v[:, k]) * v[:, k]
+        fsq = fsq + f[:, k] * f[:, k]
+        v[:, k] = (a * v[:, k] + (bom * f[:, k]) * dt) + (bom * F32(0.5)) * (beta + old_beta[:, k])
+        old_beta[:, k] = beta
+        r[:, k] = r[:, k] + v[:, k] * dt
+    r, image = apply_pbc(r, image, box)
+    return r, v, K, fsq, image, old_beta
+
+
+# ----------------------------------------------------------------------------
+# Counter-based normal variates used by the Langevin kernel.
+# Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:
+# as easy as 1, 2, 3"), then the Box-Muller transform.
+# ----------------------------------------------------------------------------
+_PHILOX_M0 = np.uint64(0xD2511F53)
+_PHILOX_M1 = np.uint64(0xCD9E8D57)
+_PHILOX_W0 = np.uint32(0x9E3779B9)
+_PHILOX_W1 = np.uint32(0xBB67AE85)
+_MASK32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Vectorised Philox-4x32-10 on uint32 arrays."""
+    c0 = np.asarray(c0, np.uint32).astype(np.uint64)
+    c1 = np.asarray(c1, np.uint32).astype(np.uint64)
+    c2 = np.asarray(c2, np.uint32).astype(np.uint64)
+    c3 = np.asarray(c3, np.uint32).astype(np.uint64)
+    k0 = np.uint64(np.uint32(k0))
+    k1 = np.uint64(np.uint32(k1))
+    for rnd in range(10):
+        p0 = _PHILOX_M0 * c0
+        p1 = _PHILOX_M1 * c2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & _MASK32
+        hi1, lo1 = p1 >> np.uint64(32), p1 & _MASK32
+        c0, c1, c2, c3 = (hi1 ^ c1 ^ k0) & _MASK32, lo1, (hi0 ^ c3 ^ k1) & _MASK32, lo0
+        if rnd < 9:
+            k0 = (k0 + np.uint64(_PHILOX_W0)) & _MASK32
+            k1 = (k1 + np.uint64(_PHILOX_W1)) & _MASK32
+    return [c.astype(np.uint32) for c in (c0, c1, c2, c3)]
+
+
+def langevin_normals(seed, step, ids):
+    """Three N(0,1) variates per particle for Langevin step ``step``.
+
+    counter = (particle id, step low, step high, 0), key = (seed low, seed high).
+    u = ((x >> 8) + 1) * 2^-24 in (0, 1];  z = sqrt(-2 ln u_a) * (cos|sin)(2 pi u_b).
+    Computed in float64 here; the device uses float32 transcendental functions,
+    so the two agree to a few float32 ulps.
+    """
+    ids = np.asarray(ids, np.int64)
+    seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    step = int(step) & 0xFFFFFFFFFFFFFFFF
+    x = philox4x32_10(ids & 0xFFFFFFFF, np.full(ids.shape, step & 0xFFFFFFFF),
+                      np.full(ids.shape, step >> 32), np.zeros(ids.shape), seed & 0xFFFFFFFF, seed >> 32)
+    u = [((xi >> np.uint32(8)).astype(F64) + 1.0) * 2.0 ** -24 for xi in x]
+    rad0 = np.sqrt(-2.0 * np.log(u[0]))
+    rad1 = np.sqrt(-2.0 * np.log(u[2]))
+    z0 = rad0 * np.cos(2 * np.pi * u[1])
+    z1 = rad0 * np.sin(2 * np.pi * u[1])
+    z2 = rad1 * np.cos(2 * np.pi * u[3])
+    return np.stack([z0, z1, z2], axis=1)
+
+
+# ----------------------------------------------------------------------------
+# Runtime actions: runtime_actions/momentum_reset.py:57-92,
+# scalar_saver.py:161-197
+# ----------------------------------------------------------------------------
+def momentum_reset(v, m):
+    """momentum_reset.py:166-186: v -= sum(m v)/sum(m), sums formed in float64."""
+    v = np.array(v, F32, copy=True)
+    m = np.asarray(m, F32)
+    tot_m = F32(np.sum(m.astype(F64)))
+    for k in range(v.shape[1]):
+        p = F32(np.sum((m * v[:, k]).astype(F64)))
+        v[:, k] = v[:, k] - p / tot_m
+    return v
+
+
+# ----------------------------------------------------------------------------
+# Radial distribution: calculators/calculator_radial_distribution.py:76-159
+# ----------------------------------------------------------------------------
+def rdf_bin_width(box, bins):
+    """calculator_radial_distribution.py:82-83: (min(L)/2)/bins, float64."""
+    half = F64(np.min(np.asarray(box, F32))) / 2
+    return half, half / bins
+
+
+def rdf_bin_of(d, box, bins):
+    """Bin of float32 distances d (:101-104), -1 when outside L_min/2."""
+    half, bw = rdf_bin_width(box, bins)
+    d = np.asarray(d, F32)
+    d64 = d.astype(F64)
+    out = np.where(d64 < half, np.floor(d64 / bw), -1).astype(np.int64)
+    return out
+
+
+def rdf_counts(r, ptype, box, bins, ntypes=None, chunk=512):
+    """Ordered pairs i != j, f32 MIC distance, dist^2 < (L/2)^2 and dist < L/2,
+    bin = int(dist/bin_width) with float64 arithmetic (:90-105).
+    Returns int64 counts of shape (nt, nt, bins)."""
+    r = np.asarray(r, F32)
+    ptype = np.asarray(ptype, np.int64)
+    N = r.shape[0]
+    nt = int(ptype.max()) + 1 if ntypes is None else ntypes
+    half, bw = rdf_bin_width(box, bins)
+    counts = np.zeros((nt, nt, bins), np.int64)
+    for s in range(0, N, chunk):
+        e = min(N, s + chunk)
+        d2 = dist_sq(r[None, :, :], r[s:e, None, :], box)  # (chunk, N) other - self symmetric
+        ii = np.arange(s, e)[:, None]
+        jj = np.arange(N)[None, :]
+        ok = (d2.astype(F64) < half * half) & (ii != jj)
+        d = np.sqrt(d2)
+        ok &= d.astype(F64) < half
+        b = np.floor(d.astype(F64) / bw).astype(np.int64)
+        ti = np.broadcast_to(ptype[s:e, None], ok.shape)[ok]
+        tj = np.broadcast_to(ptype[None, :], ok.shape)[ok]
+        np.add.at(counts, (ti, tj, b[ok]), 1)
+    return counts
+
+
+# ----------------------------------------------------------------------------
+# The timestep loop: simulation/Simulation.py:306-346 with interaction.py:99-119
+# ----------------------------------------------------------------------------
+class OracleSystem:
+    """A small-N restatement of one Simulation timeblock.
+
+    Loop of Simulation.py:312-324: steps+1 interaction evaluations, the last
+    one without integration; nblist check (nblist.py:58-86) before each
+    evaluation with r_ref starting at 0 (nblist.py:10) so the first
+    evaluation rebuilds.
+    """
+
+    def __init__(self, r, v, m, ptype, box, params, max_cut, skin, base="LJ_12_6_sigma_epsilon",
+                 cutoff="SP", integrator="NVE", dt=0.005, temperature=None, tau=None,
+                 alpha=None, seed=0, momentum_reset_every=0, image=None, nblist="N squared", save_every=0):
+        self.r = np.array(r, F32)
+        self.v = np.array(v, F32)
+        self.m = np.asarray(m, F32)
+        self.ptype = np.asarray(ptype, np.int32)
+        self.box = np.asarray(box, F32)
+        self.params = np.asarray(params, F32)
+        self.max_cut = F32(max_cut)
+        self.skin = F32(skin)
+        self.base, self.cutoff = base, cutoff
+        self.integrator = integrator
+        self.dt = F32(dt)
+        self.T = temperature
+        self.N, self.D = self.r.shape
+        self.image = np.zeros((self.N, self.D), np.int32) if image is None else np.array(image, np.int32)
+        self.r_ref = np.zeros_like(self.r)
+        self.rows = None
+        self.rebuilds = 0
+        self.xi = F32(0.0)
+        self.k_sum = F32(0.0)
+        if integrator == "NVT":
+            self.omega2 = nvt_omega2(tau)
+            self.degrees = self.N * self.D - self.D
+        self.alpha = alpha
+        self.seed = seed
+        self.old_beta = np.zeros_like(self.r)
+        self.lstep = 0
+        self.momentum_reset_every = momentum_reset_every
+        self.save_every = save_every
+        self.saved = []
+        self.nblist = nblist
+        self.f = np.zeros_like(self.r)
+        self.U = np.zeros(self.N, F32)
+        self.W = np.zeros(self.N, F32)
+        self.lap = np.zeros(self.N, F32)
+        self.K = np.zeros(self.N, F32)
+        self.fsq = np.zeros(self.N, F32)
+
+    def interactions(self):
+        if np.any(moved_beyond_half_skin(self.r, self.r_ref, self.box, self.skin)) or self.rows is None:
+            build = nblist_nsquared if self.nblist == "N squared" else nblist_linked_lists
+            self.rows = build(self.r, self.box, self.max_cut, self.skin)
+            self.r_ref = self.r.copy()
+            self.rebuilds += 1
+        self.f, self.U, self.W, self.lap = pair_forces(self.r, self.ptype, self.box, self.params, self.rows,
+                                                       self.base, self.cutoff)
+
+    def integrate(self, time):
+        if self.integrator == "NVE":
+            self.r, self.v, self.K, self.fsq, self.image = nve_step(self.r, self.v, self.f, self.m, self.image,
+                                                                   self.box, self.dt)
+        elif self.integrator == "NVT":
+            self.r, self.v, self.K, self.fsq, self.image = nvt_step(self.r, self.v, self.f, self.m, self.image,
+                                                                   self.box, self.dt, self.xi)
+            k_sum = F32(np.sum(self.K.astype(F64)))
+            self.xi = nvt_thermostat_update(self.xi, k_sum, self.dt, self.omega2, self.degrees, self.T)
+        elif self.integrator == "NVT_Langevin":
+            z = langevin_normals(self.seed, self.lstep, np.arange(self.N))
+            self.r, self.v, self.K, self.fsq, self.image, self.old_beta = langevin_step(
+                self.r, self.v, self.f, self.m, self.image, self.box, self.dt, self.alpha, self.T, z,
+                self.old_beta)
+            self.lstep += 1
+        else:
+            raise ValueError(self.integrator)
+
+    def run_block(self, steps, time_zero=0.0):
+        for step in range(steps + 1):
+            time = F64(F32(time_zero)) + step * F64(self.dt)
+            self.interactions()
+            if step < steps:
+                self.integrate(time)
+                if self.momentum_reset_every and step % self.momentum_reset_every == 0:
+                    self.v = momentum_reset(self.v, self.m)
+                if self.save_every and step % self.save_every == 0:
+                    # scalar_saver.py:378-392: sums of U, W, lapU, K, Fsq at this step
+                    self.saved.append([np.sum(x.astype(F64)) for x in (self.U, self.W, self.lap, self.K, self.fsq)])

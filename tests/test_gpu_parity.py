@@ -1,0 +1,249 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the rumd3 goldens.
+
+Tolerances (float32 path):
+  forces     max|F_gpu - F_ref| <= 1e-5 * max|F_ref|      (vs the float32 oracle)
+             and <= 2e-5 * max|F| vs rumd3 and the float64 brute force
+  energies   |U_gpu - U_ref| <= 1e-5 * (1 + |U_ref|) per particle
+  neighbour sets and RDF counts: bit-exact
+  trajectories (tens of steps): positions within 1e-4, velocities within 1e-3
+"""
+import numpy as np
+import pytest
+
+import gamdpy_amd as gp
+from gamdpy_amd import _native
+from oracle import md_oracle as orc
+from helpers import (lj_fcc, ka_fcc, ka_pair_potential, lj_pair_potential, rumd3_conf, plan, rows_from_array,
+                     KA_SIG, KA_EPS)
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def test_device_is_mi355x():
+    info = _native.device_info()
+    print(info)
+    assert info["arch"].startswith("gfx950")
+    assert info["wavefront_size"] == 64
+
+
+@pytest.mark.parametrize("cutoff", ["SP", "SF", None])
+@pytest.mark.parametrize("tp", [1, 4, 16])
+def test_evaluator_lj_nsquared(tp, cutoff):
+    conf = lj_fcc([6, 6, 6], jitter=0.2, flags={'lapU': True})
+    pp = lj_pair_potential(cutoff=cutoff, max_nbs=200)
+    ev = gp.Evaluator(conf, pp, compute_plan=plan(tp=tp))
+    ev.evaluate()
+    table = pp.param_table()[0]
+    rows = orc.nblist_nsquared(conf['r'], conf.simbox.lengths, 2.5, 0.5)
+    f, U, W, lap = orc.pair_forces(conf['r'], conf.ptype, conf.simbox.lengths, table, rows, cutoff=cutoff)
+    assert _rel(conf['f'], f) < 1e-5
+    np.testing.assert_allclose(conf['U'], U, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(conf['W'], W, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(conf['lapU'], lap, rtol=2e-5, atol=1e-3)
+    got = rows_from_array(ev.ctx.nblist())
+    for a, b in zip(got, rows):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("nblist", ["N squared", "linked lists"])
+@pytest.mark.parametrize("tp", [1, 4])
+def test_nblist_sets_bit_exact(nblist, tp):
+    """Reference test_linked_lists_nblist.py:46-107: FCC jittered by +-0.3 and
+    permuted; the lists must hold exactly the oracle's pairs."""
+    conf = lj_fcc([8, 8, 8])
+    np.random.seed(0)
+    conf['r'] = conf['r'] + np.random.uniform(-.3, +.3, conf['r'].shape)
+    conf['r'] = conf['r'][np.random.permutation(conf.N), :]
+    pp = lj_pair_potential(max_nbs=300)
+    ev = gp.Evaluator(conf, pp, compute_plan=plan(tp=tp, nblist=nblist, skin=0.5))
+    ev.evaluate()
+    arr = ev.ctx.nblist()
+    rows = orc.nblist_nsquared(conf['r'], conf.simbox.lengths, 2.5, 0.5)
+    assert np.array_equal(arr[:, -1], [len(x) for x in rows])
+    for a, b in zip(rows_from_array(arr), rows):
+        np.testing.assert_array_equal(a, b)
+    assert ev.ctx.nbflag()[1] == 0
+
+
+def test_ka_rumd3_golden_forces():
+    """rumd3 (independent code) forces, pe and virial of a KA N=4000 snapshot."""
+    for k in (0, 1):
+        conf, d = rumd3_conf(k)
+        pp = ka_pair_potential(max_nbs=400)
+        ev = gp.Evaluator(conf, pp, compute_plan=plan(tp=4, nblist='linked lists', skin=0.3))
+        ev.evaluate()
+        assert _rel(conf['f'], d['f']) < 2e-5
+        assert np.max(np.abs(conf['U'] - d['pe'])) < 2e-4
+        assert _rel(3.0 * conf['W'], d['vir']) < 2e-5
+        # float32 oracle on the same list
+        table = pp.param_table()[0]
+        rows = orc.nblist_nsquared(conf['r'], conf.simbox.lengths, 2.5, 0.3)
+        f, U, W, _ = orc.pair_forces(conf['r'], conf.ptype, conf.simbox.lengths, table, rows)
+        assert _rel(conf['f'], f) < 1e-5
+        np.testing.assert_allclose(conf['U'], U, rtol=1e-5, atol=2e-5)
+        # float64 brute force: the error of the float32 path
+        f64, U64, W64 = orc.pair_forces_f64(conf['r'], conf.ptype, conf.simbox.lengths, table)
+        err64 = _rel(conf['f'], f64)
+        print(f"snapshot {k}: rel force error vs float64 = {err64:.2e}")
+        assert err64 < 2e-5
+
+
+def _oracle_for(conf, pp, integrator, **kw):
+    table, max_cut = pp.param_table()
+    return orc.OracleSystem(conf['r'].copy(), conf['v'].copy(), conf['m'].copy(), conf.ptype.copy(),
+                            conf.simbox.lengths, table, max_cut, kw.pop('skin', 0.5), integrator=integrator, **kw)
+
+
+@pytest.mark.parametrize("driver", ["gridsync", "graph", "plain"])
+def test_nve_trajectory(driver):
+    conf = lj_fcc([5, 5, 5], jitter=0.05, flags={'lapU': True, 'Fsq': True})
+    pp = lj_pair_potential(max_nbs=200)
+    o = _oracle_for(conf, pp, "NVE", dt=0.005)
+    p = plan(tp=4, gridsync=driver == "gridsync", graph=driver == "graph")
+    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.005), [], num_timeblocks=1, steps_per_timeblock=64,
+                        compute_plan=p, storage='memory', steps_in_kernel_test=0)
+    for _ in sim.run_timeblocks():
+        pass
+    o.run_block(64)
+    assert np.max(np.abs(conf['r'] - o.r)) < 1e-4
+    assert np.max(np.abs(conf['v'] - o.v)) < 1e-3
+    assert np.array_equal(conf.r_im, o.image)
+    np.testing.assert_allclose(conf['U'], o.U, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(conf['K'], o.K, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(conf['Fsq'], o.fsq, rtol=1e-3, atol=1e-2)
+    assert pp.nblist.d_nbflag.copy_to_host()[2] == o.rebuilds
+
+
+def test_drivers_bitwise_identical():
+    """Persistent, graph and plain launches run the same kernels on the same
+    block decomposition: NVT results are identical to the last bit."""
+    out = []
+    for driver in ("gridsync", "graph", "plain"):
+        conf = ka_fcc([5, 5, 5], T=1.0, jitter=0.05)
+        pp = ka_pair_potential(max_nbs=300)
+        integ = gp.NVT(temperature=0.8, tau=0.2, dt=0.004)
+        sim = gp.Simulation(conf, pp, integ, [gp.MomentumReset(7)], num_timeblocks=2, steps_per_timeblock=40,
+                            compute_plan=plan(tp=4, gridsync=driver == "gridsync", graph=driver == "graph"),
+                            storage='memory')
+        for _ in sim.run_timeblocks():
+            pass
+        out.append((conf['r'].copy(), conf['v'].copy(), conf['f'].copy(), integ.thermostat_state.copy()))
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_nvt_trajectory_and_thermostat():
+    conf = ka_fcc([5, 5, 5], T=1.0, jitter=0.05)
+    pp = ka_pair_potential(max_nbs=300)
+    o = _oracle_for(conf, pp, "NVT", dt=0.004, temperature=0.8, tau=0.2, skin=0.3)
+    integ = gp.NVT(temperature=0.8, tau=0.2, dt=0.004)
+    sim = gp.Simulation(conf, pp, integ, [], num_timeblocks=1, steps_per_timeblock=60,
+                        compute_plan=plan(tp=4, gridsync=True, skin=0.3), storage='memory', steps_in_kernel_test=0)
+    for _ in sim.run_timeblocks():
+        pass
+    o.run_block(60)
+    assert np.max(np.abs(conf['r'] - o.r)) < 1e-4
+    assert np.max(np.abs(conf['v'] - o.v)) < 1e-3
+    assert abs(float(integ.thermostat_state[0]) - float(o.xi)) <= 1e-4 * max(1.0, abs(float(o.xi)))
+
+
+def test_langevin_trajectory():
+    conf = lj_fcc([5, 5, 5], jitter=0.05)
+    pp = lj_pair_potential(max_nbs=200)
+    o = _oracle_for(conf, pp, "NVT_Langevin", dt=0.005, temperature=0.7, alpha=0.1, seed=213)
+    integ = gp.NVT_Langevin(temperature=0.7, alpha=0.1, dt=0.005, seed=213)
+    sim = gp.Simulation(conf, pp, integ, [], num_timeblocks=1, steps_per_timeblock=40,
+                        compute_plan=plan(tp=4, gridsync=True), storage='memory', steps_in_kernel_test=0)
+    for _ in sim.run_timeblocks():
+        pass
+    o.run_block(40)
+    assert np.max(np.abs(conf['r'] - o.r)) < 1e-4
+    assert np.max(np.abs(conf['v'] - o.v)) < 1e-3
+
+
+def test_scalar_saver_and_momentum_reset():
+    flags = {'lapU': True, 'Fsq': True}
+    conf = lj_fcc([5, 5, 5], jitter=0.05, flags=flags)
+    pp = lj_pair_potential(max_nbs=200)
+    o = _oracle_for(conf, pp, "NVE", dt=0.005, momentum_reset_every=10, save_every=8)
+    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.005), [gp.MomentumReset(10), gp.ScalarSaver(8, flags)],
+                        num_timeblocks=2, steps_per_timeblock=64, compute_plan=plan(tp=4, gridsync=True),
+                        storage='memory', steps_in_kernel_test=0)
+    for _ in sim.run_timeblocks():
+        pass
+    o.run_block(64)
+    o.run_block(64)
+    U, W, lap, K, fsq = gp.extract_scalars(sim.output, ['U', 'W', 'lapU', 'K', 'Fsq'])
+    ref = np.array(o.saved)
+    assert len(U) == len(ref) == 16
+    np.testing.assert_allclose(U, ref[:, 0], rtol=1e-4)
+    np.testing.assert_allclose(W, ref[:, 1], rtol=1e-4, atol=1e-2)
+    np.testing.assert_allclose(lap, ref[:, 2], rtol=1e-4)
+    np.testing.assert_allclose(K, ref[:, 3], rtol=1e-4)
+    np.testing.assert_allclose(fsq, ref[:, 4], rtol=1e-3)
+    # total momentum is zero after the final reset (step 60 of the last block)
+    p = np.sum(conf['v'] * conf['m'][:, None], axis=0)
+    assert np.all(np.abs(p) < 1e-3)
+    assert np.max(np.abs(conf['v'] - o.v)) < 1e-3
+
+
+def test_rdf_counts_bit_exact():
+    conf, d = rumd3_conf(0)
+    calc = gp.CalculatorRadialDistribution(conf, bins=1000)
+    calc.update()
+    ref = orc.rdf_counts(conf['r'], conf.ptype, conf.simbox.lengths, 1000)
+    got = calc.rdf_list[0]
+    assert np.array_equal(got.astype(np.int64), ref)
+    assert got.sum() == ref.sum()
+
+
+def test_rdf_from_simulation_context_matches_host():
+    conf = ka_fcc([6, 6, 6], T=1.0)
+    pp = ka_pair_potential(max_nbs=300)
+    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.004), [], num_timeblocks=1, steps_per_timeblock=20,
+                        compute_plan=plan(tp=4), storage='memory')
+    calc = gp.CalculatorRadialDistribution(conf, bins=300)
+    for _ in sim.run_timeblocks():
+        calc.update()
+    ref = orc.rdf_counts(conf['r'], conf.ptype, conf.simbox.lengths, 300)
+    assert np.array_equal(calc.rdf_list[0].astype(np.int64), ref)
+
+
+def test_rdf_ideal_gas():
+    """Reference test_radial_distrubution.py:9-37 (D=3): mean g(r) = 1 +- 0.01."""
+    for rho in (1.0, 0.32):
+        N = 10_000
+        conf = gp.Configuration(D=3)
+        conf.make_positions(N=N, rho=rho)
+        conf['m'] = 1.0
+        calc = gp.CalculatorRadialDistribution(conf, bins=16)
+        rng = np.random.default_rng(3)
+        for _ in range(4):
+            conf['r'] = (rng.random((N, 3)) - 0.5) * conf.simbox.lengths
+            calc.update()
+        data = calc.read()
+        assert data['rdf'].shape == (4, 16)
+        assert abs(np.mean(data['rdf']) - 1.0) < 0.01
+        assert abs(np.max(data['rdf']) - 1.0) < 0.8
+
+
+def test_overflow_raises():
+    conf = lj_fcc([6, 6, 6])
+    pp = lj_pair_potential(max_nbs=20)
+    with pytest.raises(RuntimeError, match="Neighbor-list is invalid"):
+        gp.Simulation(conf, pp, gp.NVE(dt=0.005), [], num_timeblocks=1, steps_per_timeblock=8,
+                      compute_plan=plan(tp=4), storage='memory')
+
+
+def test_unsupported_plans_raise():
+    conf = lj_fcc([4, 4, 4])
+    pp = lj_pair_potential()
+    with pytest.raises(_native.NativeError):
+        gp.Evaluator(conf, pp, compute_plan=dict(plan(tp=4), UtilizeNIII=True)).evaluate()
+    with pytest.raises((AssertionError, _native.NativeError)):
+        gp.Evaluator(conf, pp, compute_plan=plan(tp=4, nblist='linked lists'))
