@@ -13,10 +13,11 @@
 #include "gmd_kernels.h"
 
 // ============================================================================
-// Kernels: persistent (cooperative) driver
+// Drivers
 // ============================================================================
 template <int BASE, int CUT, int INTEG>
-__global__ void __launch_bounds__(1024) k_persistent(DP P) {
+__global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
+    const DP &P = *Pp;
     extern __shared__ __align__(16) char lds[];
     PairP *pp = (PairP *)(lds + P.lds_pp);
     double *red = (double *)(lds + P.lds_red);
@@ -24,138 +25,57 @@ __global__ void __launch_bounds__(1024) k_persistent(DP P) {
     load_pair_table(P, pp);
     __syncthreads();
     const int nsteps = P.ctrl->nsteps;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gstride = gridDim.x * blockDim.x;
-    const int wpb = blockDim.x >> 6;
-    const int wave_global = blockIdx.x * wpb + (threadIdx.x >> 6);
-    const int wave_stride = gridDim.x * wpb;
     for (int s = 0; s <= nsteps; ++s) {
-        if (ld_flag(&P.rebuild_flag[s])) {
-            const float4 *pos = cur_pos(P, s);
-            if (P.nblist_kind == 1) {
-                ll_assign(P, pos, gtid, gstride);
-                grid_barrier(P.ctrl, NoTail());
-                if (blockIdx.x == 0) ll_scan_block(P, (int *)build);
-                grid_barrier(P.ctrl, NoTail());
-                ll_scatter(P, gtid, gstride);
-                grid_barrier(P.ctrl, NoTail());
-                ll_sort_cells(P, gtid, gstride);
-                grid_barrier(P.ctrl, NoTail());
-                ll_build(P, pos, build, wave_global, wave_stride);
-            } else {
-                nsq_build(P, pos, (float4 *)build);
-            }
-            if (gtid == 0) P.ctrl->nbflag[2] += 1;
-            grid_barrier(P.ctrl, NoTail());
-        }
-        const StepInfo I = step_info(P, s, INTEG);
+        const bool rebuild = ld_flag(P.rebuild_flag + s) != 0;
+        const StepInfo I = step_info(P, s, INTEG, rebuild);
         float acc[GMD_NRED];
 #pragma unroll
         for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
-        phase_step<BASE, CUT, INTEG>(P, I, pp, acc);
+        step_body<BASE, CUT, INTEG>(P, I, pp, build, acc);
         if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, acc, red);
         grid_barrier(P.ctrl, [&]() { step_tail(P, I, red); });
     }
 }
 
-// ============================================================================
-// Kernels: kernel-per-phase driver (each reads the step from the control block)
-// ============================================================================
-__device__ __forceinline__ bool rebuild_active(const DP &P, int &s) {
-    s = P.ctrl->step;
-    if (s > P.ctrl->nsteps) return false;
-    return ld_flag(&P.rebuild_flag[s]) != 0;
-}
-
-__global__ void k_ll_assign(DP P) {
-    int s;
-    if (!rebuild_active(P, s)) return;
-    ll_assign(P, cur_pos(P, s), blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-
-__global__ void __launch_bounds__(1024) k_ll_scan(DP P) {
-    extern __shared__ __align__(16) char lds[];
-    int s;
-    if (!rebuild_active(P, s)) return;
-    ll_scan_block(P, (int *)lds);
-}
-
-__global__ void k_ll_scatter(DP P) {
-    int s;
-    if (!rebuild_active(P, s)) return;
-    ll_scatter(P, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-
-__global__ void k_ll_sort(DP P) {
-    int s;
-    if (!rebuild_active(P, s)) return;
-    ll_sort_cells(P, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-
-__global__ void __launch_bounds__(1024) k_ll_build(DP P) {
-    extern __shared__ __align__(16) char lds[];
-    int s;
-    if (!rebuild_active(P, s)) return;
-    const int wpb = blockDim.x >> 6;
-    ll_build(P, cur_pos(P, s), lds, blockIdx.x * wpb + (threadIdx.x >> 6), gridDim.x * wpb);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&P.ctrl->nbflag[2], 1);
-}
-
-__global__ void __launch_bounds__(1024) k_nsq_build(DP P) {
-    extern __shared__ __align__(16) char lds[];
-    int s;
-    if (!rebuild_active(P, s)) return;
-    nsq_build(P, cur_pos(P, s), (float4 *)lds);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&P.ctrl->nbflag[2], 1);
-}
-
+// One step per launch; the step index and the rebuild decision come from the
+// control block written by the previous launch's last block.
 template <int BASE, int CUT, int INTEG>
-__global__ void __launch_bounds__(1024) k_step(DP P) {
+__global__ void __launch_bounds__(256) k_step(const DP *__restrict__ Pp) {
+    const DP &P = *Pp;
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_last;
     const int s = P.ctrl->step;
     if (s > P.ctrl->nsteps) return;
+    const bool rebuild = P.ctrl->rebuild_now != 0;
     PairP *pp = (PairP *)(lds + P.lds_pp);
     double *red = (double *)(lds + P.lds_red);
+    char *build = lds + P.lds_build;
     load_pair_table(P, pp);
     __syncthreads();
-    const StepInfo I = step_info(P, s, INTEG);
+    const StepInfo I = step_info(P, s, INTEG, rebuild);
     float acc[GMD_NRED];
 #pragma unroll
     for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
-    phase_step<BASE, CUT, INTEG>(P, I, pp, acc);
+    step_body<BASE, CUT, INTEG>(P, I, pp, build, acc);
     if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, acc, red);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned a = atomicAdd(&P.ctrl->red_count, 1u);
+        const unsigned a =
+            __hip_atomic_fetch_add(gen_ptr(&P.ctrl->red_count), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (a == gridDim.x - 1);
     }
     __syncthreads();
     if (s_last) {
-        __threadfence();
         step_tail(P, I, red);
-        if (threadIdx.x == 0) {
-            P.ctrl->red_count = 0;
-            P.ctrl->step = s + 1;
-            __threadfence();
-        }
+        if (threadIdx.x == 0) P.ctrl->red_count = 0;
     }
 }
 
-// Skin test of the positions a run starts from (nblist.py:58-86 at step 0):
-// a rebuild is due when any particle is more than skin/2 from r_ref.
-__global__ void k_skin_check(DP P) {
-    const float4 *pos = cur_pos(P, 0);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += gridDim.x * blockDim.x) {
-        const float4 r = pos[i], rr = P.rref[i];
-        float dx, dy, dz;
-        if (dist_sq_mic(r.x, r.y, r.z, rr.x, rr.y, rr.z, P.box0, P.box1, P.box2, dx, dy, dz) > P.skin2q)
-            P.rebuild_flag[0] = 1;
-    }
-}
-
-__global__ void k_begin_run(Ctrl *C, int nsteps, float t0, int buf0, long long lstep0) {
+// ---- run set-up: cell grid of the starting positions and the skin test ----
+// Kernel parameters are generic pointers (an address-space qualifier would
+// change the kernel's mangled name between the host and device passes).
+__global__ void k_begin_run(Ctrl *Cp, int nsteps, float t0, int buf0, long long lstep0) {
+    GMEM Ctrl *C = (GMEM Ctrl *)Cp;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         C->nsteps = nsteps;
         C->t0 = t0;
@@ -164,7 +84,33 @@ __global__ void k_begin_run(Ctrl *C, int nsteps, float t0, int buf0, long long l
         C->step = 0;
         C->red_count = 0;
         C->bar_count = 0;
+        C->cell_overflow = 0;
     }
+}
+
+__global__ void __launch_bounds__(256) k_cells_clear(const DP *__restrict__ Pp) {
+    const DP &P = *Pp;
+    const int n = 3 * P.ncells;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) P.ccount[c] = 0;
+}
+
+// skin test of the starting positions (nblist.py:58-86 at step 0) and, for
+// linked lists, insertion into cell buffer 0
+__global__ void __launch_bounds__(256) k_run_setup(const DP *__restrict__ Pp) {
+    const DP &P = *Pp;
+    cpos_t pos = P.ctrl->buf0 ? P.pos1 : P.pos0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += gridDim.x * blockDim.x) {
+        const float4 r = pos[i], rr = P.rref[i];
+        float dx, dy, dz;
+        if (dist_sq_mic(r.x, r.y, r.z, rr.x, rr.y, rr.z, P.box0, P.box1, P.box2, dx, dy, dz) > P.skin2q)
+            P.rebuild_flag[0] = 1;
+        if (P.nblist_kind == 1) cell_insert(P, 0, i, r.x, r.y, r.z);
+    }
+}
+
+__global__ void k_setup_finish(const DP *__restrict__ Pp) {
+    const DP &P = *Pp;
+    if (threadIdx.x == 0 && blockIdx.x == 0) P.ctrl->rebuild_now = P.rebuild_flag[0];
 }
 
 // ============================================================================
@@ -174,36 +120,37 @@ struct RdfP {
     int N, nt, bins, nclass, tile, ntiles;
     long long npairs;
     float L0, L1, L2, inv_bw;
-    const float4 *pos;
-    const int *ptype;   // nullptr: type from pos.w
-    const float *thr2;  // bins+1 thresholds on the squared distance
-    unsigned *part;     // [gridDim][nclass*bins]
+    cpos_t pos;
+    const GMEM int *ptype;     // nullptr: type from pos.w
+    const GMEM float2 *thr2;   // (thr[k], thr[k+1]) k = 0..bins-1; thr = bin thresholds on d^2
+    float out_thr;             // smallest excluded d^2 (d >= min(L)/2)
+    GMEM unsigned *part;       // [gridDim][nclass*bins]
 };
-
-__device__ __forceinline__ int type_class(int a, int b, int nt) {
-    if (a > b) {
-        int t = a;
-        a = b;
-        b = t;
-    }
-    return a * nt - (a * (a - 1)) / 2 + (b - a);
-}
 
 // Unordered pairs i < j are visited once; each adds one count to the class
 // {ti, tj}.  Ordered counts are recovered on the host (the MIC distance is
-// symmetric bit for bit, so d(i,j) == d(j,i)).
+// symmetric bit for bit, so d(i,j) == d(j,i)).  The bin of a pair is found
+// without a square root: a fast estimate k0 from v_sqrt_f32 is corrected
+// against exact thresholds on d^2 (thr[k] = smallest float32 d^2 whose bin
+// under the reference's float64 rule is >= k), so the counts are exact.
 __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
     extern __shared__ __align__(16) char lds[];
     const int nh = R.nclass * R.bins;
     unsigned *hist = (unsigned *)lds;
-    float *thr = (float *)(hist + nh);
-    float4 *tj = (float4 *)(lds + (((size_t)nh * 4 + (size_t)(R.bins + 1) * 4 + 15) & ~(size_t)15));
-    int *tt = (int *)(tj + R.tile);
+    float2 *thr = (float2 *)(lds + (((size_t)nh * 4 + 15) & ~(size_t)15));
+    float4 *tj = (float4 *)(thr + R.bins);
+    int *cls = (int *)(tj + R.tile);   // class offset table [nt*nt]
     for (int x = threadIdx.x; x < nh; x += blockDim.x) hist[x] = 0u;
-    for (int x = threadIdx.x; x <= R.bins; x += blockDim.x) thr[x] = R.thr2[x];
+    for (int x = threadIdx.x; x < R.bins; x += blockDim.x) thr[x] = R.thr2[x];
+    for (int x = threadIdx.x; x < R.nt * R.nt; x += blockDim.x) {
+        int a = x / R.nt, b = x % R.nt;
+        int lo = min(a, b), hi = max(a, b);
+        cls[x] = (lo * R.nt - (lo * (lo - 1)) / 2 + (hi - lo)) * R.bins;
+    }
     __syncthreads();
-    const float out_thr = thr[R.bins];
+    const float out_thr = R.out_thr;
     const long long T = R.ntiles;
+    const int bmax = R.bins - 1;
     for (long long p = blockIdx.x; p < R.npairs; p += gridDim.x) {
         // decode p -> (I, J), J >= I, rows of length T - I
         double a = (double)(2 * T + 1);
@@ -216,10 +163,11 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
         __syncthreads();
         const int j0 = (int)(J * R.tile);
         for (int x = threadIdx.x; x < R.tile; x += blockDim.x) {
-            int j = j0 + x;
+            const int j = j0 + x;
             if (j < R.N) {
-                tj[x] = R.pos[j];
-                tt[x] = R.ptype ? R.ptype[j] : __float_as_int(R.pos[j].w);
+                float4 rj = R.pos[j];
+                if (R.ptype) rj.w = __int_as_float(R.ptype[j]);
+                tj[x] = rj;
             }
         }
         __syncthreads();
@@ -227,6 +175,7 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
         if (threadIdx.x < R.tile && i < R.N) {
             const float4 ri = R.pos[i];
             const int ti = R.ptype ? R.ptype[i] : __float_as_int(ri.w);
+            const int *clsi = cls + ti * R.nt;
             const int jn = min(R.tile, R.N - j0);
             const int jstart = (I == J) ? (threadIdx.x + 1) : 0;
             for (int jj = jstart; jj < jn; ++jj) {
@@ -234,26 +183,31 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
                 float dx, dy, dz;
                 const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, R.L0, R.L1, R.L2, dx, dy, dz);
                 if (d2 < out_thr) {
-                    int k = (int)(sqrtf(d2) * R.inv_bw);
-                    k = max(0, min(k, R.bins - 1));
-                    while (k > 0 && d2 < thr[k]) --k;
-                    while (k + 1 < R.bins && d2 >= thr[k + 1]) ++k;
-                    atomicAdd(&hist[type_class(ti, tt[jj], R.nt) * R.bins + k], 1u);
+                    int k = (int)(__builtin_amdgcn_sqrtf(d2) * R.inv_bw);
+                    k = min(k, bmax);
+                    const float2 th = thr[k];
+                    k = k - (d2 < th.x ? 1 : 0) + (d2 >= th.y ? 1 : 0);
+                    k = max(0, min(k, bmax));
+                    atomicAdd(&hist[clsi[__float_as_int(rj.w)] + k], 1u);
                 }
             }
         }
     }
     __syncthreads();
-    unsigned *dst = R.part + (size_t)blockIdx.x * nh;
+    GMEM unsigned *dst = R.part + (size_t)blockIdx.x * nh;
     for (int x = threadIdx.x; x < nh; x += blockDim.x) dst[x] = hist[x];
 }
 
-__global__ void k_rdf_reduce(const unsigned *part, int nblocks, int nh, unsigned long long *out) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum the per-block histograms: thread (x, chunk) adds 'per' blocks of bin x.
+__global__ void k_rdf_reduce(const unsigned *part_p, int nblocks, int nh, int per, unsigned long long *out_p) {
+    const GMEM unsigned *part = (const GMEM unsigned *)part_p;
+    GMEM unsigned long long *out = (GMEM unsigned long long *)out_p;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nh) return;
+    const int b0 = blockIdx.y * per, b1 = min(nblocks, b0 + per);
     unsigned long long s = 0;
-    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nh + x];
-    out[x] = s;
+    for (int b = b0; b < b1; ++b) s += part[(size_t)b * nh + x];
+    if (s) atomicAdd(gen_ptr(out + x), s);
 }
 
 __global__ void k_pack_pos(const float *r, const int *ptype, int N, float4 *pos) {
@@ -292,7 +246,7 @@ int dalloc(T **p, size_t n) {
     return 0;
 }
 
-typedef void (*kfun)(DP);
+typedef void (*kfun)(const DP *);
 
 template <int BASE, int CUT, int INTEG>
 void pick(kfun &pers, kfun &step) {
@@ -348,7 +302,7 @@ int ilog2(int x) {
 
 }  // namespace
 
-static const int GRAPH_STEPS = 8;
+static const int GRAPH_STEPS = 20;
 
 struct gmd_ctx {
     gmd_desc d;
@@ -363,7 +317,7 @@ struct gmd_ctx {
     float4 *pos0 = nullptr, *pos1 = nullptr, *vel = nullptr, *frc = nullptr, *rref = nullptr, *obeta = nullptr;
     float *sc = nullptr;
     int *rim = nullptr, *nbl = nullptr, *nbc = nullptr;
-    int *cell_of = nullptr, *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_members = nullptr;
+    int *ccount = nullptr, *cmem = nullptr;   // cell grid, three rotating buffers
     int *d_excl = nullptr;
     float *d_params = nullptr;
     Ctrl *ctrl = nullptr;
@@ -379,6 +333,7 @@ struct gmd_ctx {
     float last_ms = 0.f;
     // launch
     DP P;
+    DP *d_P = nullptr;            // device copy of P (kernels take a pointer)
     kfun k_pers = nullptr, k_stp = nullptr;
     int block = 256, grid_pers = 1, grid_step = 1, grid_aux = 1;
     size_t lds_pers = 0, lds_step = 0, lds_build = 0;
@@ -427,8 +382,8 @@ static int ctx_free(gmd_ctx *c) {
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     void *ptrs[] = {c->pos0, c->pos1, c->vel, c->frc, c->rref, c->obeta, c->sc, c->rim, c->nbl, c->nbc,
-                    c->cell_of, c->cell_rank, c->cell_count, c->cell_start, c->cell_members, c->d_excl,
-                    c->d_params, c->ctrl, c->partial, c->rebuild_flag, c->saver_out};
+                    c->ccount, c->cmem, c->d_excl,
+                    c->d_params, c->ctrl, c->partial, c->rebuild_flag, c->saver_out, c->d_P};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -514,18 +469,27 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     const int tp = d.tp;
     int block = d.pb * tp;
     if (block % 64) block = ((block + 63) / 64) * 64;
-    if (block > 1024) block = 1024;
+    if (block > 256) block = 256;
     c->block = block;
     const int slots = block / tp;
     c->grid_step = (N + slots - 1) / slots;
     c->grid_aux = std::max(1, std::min((N + block - 1) / block, c->cus * 16));
     const int wpb = block / 64;
-    int llcap = 512;
-    while (llcap > 64 && (size_t)wpb * llcap * 16 > 64 * 1024) llcap >>= 1;
-    c->lds_build = std::max((size_t)wpb * llcap * 16, (size_t)block * 16);
+    // dynamic LDS: pair table (4 KB) | reduction scratch (2 KB) | rebuild scratch
+    c->lds_build = std::max((size_t)wpb * LL_WAVE_LDS, (size_t)block * 16);
     const size_t lds_pp = 0, lds_red = 4096, lds_b = 4096 + 16 * 16 * 8;
-    c->lds_step = lds_b;
-    c->lds_pers = lds_b + c->lds_build;
+    c->lds_step = lds_b + c->lds_build;
+    c->lds_pers = c->lds_step;
+    // cell capacity: room for 4x the mean occupancy, at least 16, at most 64
+    int ccap = 16;
+    if (d.nblist_kind == GMD_NBLIST_LINKED_LISTS) {
+        const double mean = (double)N / ncells;
+        while (ccap < 4.0 * mean + 8.0 && ccap < LL_OWNCAP) ccap *= 2;
+        if (4.0 * mean + 8.0 > LL_OWNCAP) {
+            ctx_free(c);
+            return fail("linked lists: %.1f particles per cell on average exceeds the cell capacity %d", mean, LL_OWNCAP);
+        }
+    }
 
     // ---- buffers ----
     const size_t rows = (size_t)(d.max_nbs + tp - 1) / tp;
@@ -540,11 +504,8 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     TRY(dalloc(&c->rim, (size_t)3 * N));
     TRY(dalloc(&c->nbl, c->nbl_len));
     TRY(dalloc(&c->nbc, N));
-    TRY(dalloc(&c->cell_of, N));
-    TRY(dalloc(&c->cell_rank, N));
-    TRY(dalloc(&c->cell_count, ncells));
-    TRY(dalloc(&c->cell_start, (size_t)ncells + 1));
-    TRY(dalloc(&c->cell_members, N));
+    TRY(dalloc(&c->ccount, (size_t)3 * ncells));
+    TRY(dalloc(&c->cmem, (size_t)3 * ncells * ccap));
     TRY(dalloc(&c->d_params, c->params.size()));
     TRY(dalloc(&c->ctrl, 1));
     const int max_grid = std::max(c->grid_step, c->cus * 32);
@@ -555,7 +516,7 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
         HTRY(hipMemcpy(c->d_excl, c->excl.data(), c->excl.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     HTRY(hipMemcpy(c->d_params, c->params.data(), c->params.size() * sizeof(float), hipMemcpyHostToDevice));
-    HTRY(hipMemset(c->cell_count, 0, (size_t)ncells * sizeof(int)));
+    HTRY(hipMemset(c->ccount, 0, (size_t)3 * ncells * sizeof(int)));
     HTRY(hipMemset(c->sc, 0, (size_t)5 * N * sizeof(float)));
     HTRY(hipMemset(c->rim, 0, (size_t)3 * N * sizeof(int)));
     HTRY(hipMemset(c->nbc, 0, (size_t)N * sizeof(int)));
@@ -589,29 +550,26 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
         P.cps2 = cps * cps;
         P.skin2q = (d.skin * d.skin) * 0.25f;
     }
-    P.pos0 = c->pos0;
-    P.pos1 = c->pos1;
-    P.vel = c->vel;
-    P.frc = c->frc;
-    P.rref = c->rref;
-    P.obeta = c->obeta;
-    P.sc = c->sc;
-    P.rim = c->rim;
-    P.nbl = c->nbl;
-    P.nbc = c->nbc;
+    P.pos0 = (GMEM float4 *)c->pos0;
+    P.pos1 = (GMEM float4 *)c->pos1;
+    P.vel = (GMEM float4 *)c->vel;
+    P.frc = (GMEM float4 *)c->frc;
+    P.rref = (GMEM float4 *)c->rref;
+    P.obeta = (GMEM float4 *)c->obeta;
+    P.sc = (GMEM float *)c->sc;
+    P.rim = (GMEM int *)c->rim;
+    P.nbl = (GMEM int *)c->nbl;
+    P.nbc = (GMEM int *)c->nbc;
     P.ncx = ncx;
     P.ncy = ncy;
     P.ncz = ncz;
     P.ncells = ncells;
-    P.llcap = llcap;
-    P.cell_of = c->cell_of;
-    P.cell_rank = c->cell_rank;
-    P.cell_count = c->cell_count;
-    P.cell_start = c->cell_start;
-    P.cell_members = c->cell_members;
-    P.excl = c->d_excl;
+    P.ccap = ccap;
+    P.ccount = (GMEM int *)c->ccount;
+    P.cmem = (GMEM int *)c->cmem;
+    P.excl = (const GMEM int *)c->d_excl;
     P.excl_cols = d.exclusions_cols;
-    P.raw_params = c->d_params;
+    P.raw_params = (const GMEM float *)c->d_params;
     P.pair_base = d.pair_base;
     P.cutoff = d.cutoff;
     P.dt = d.dt;
@@ -632,13 +590,14 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     P.mom_every = d.momentum_reset_every;
     P.save_every = d.scalar_save_every;
     P.saves_per_block = d.scalar_saves_per_block;
-    P.ctrl = c->ctrl;
-    P.partial = c->partial;
-    P.saver_out = c->saver_out;
+    P.ctrl = (GMEM Ctrl *)c->ctrl;
+    P.partial = (GMEM float *)c->partial;
+    P.saver_out = (GMEM float *)c->saver_out;
     P.lds_pp = (int)lds_pp;
     P.lds_red = (int)lds_red;
     P.lds_build = (int)lds_b;
 
+    TRY(dalloc(&c->d_P, 1));
     pick_kernels(d.pair_base, d.cutoff, d.integrator, c->k_pers, c->k_stp);
 
     // ---- cooperative grid: all blocks must be co-resident ----
@@ -773,7 +732,8 @@ static int ensure_flags(gmd_ctx *c, int steps) {
         int cap = std::max(steps + 2, 1024);
         if (dalloc(&c->rebuild_flag, cap)) return -1;
         c->rebuild_cap = cap;
-        c->P.rebuild_flag = c->rebuild_flag;
+        c->P.rebuild_flag = (GMEM int *)c->rebuild_flag;
+        HIPCHECK(hipMemcpy(c->d_P, &c->P, sizeof(DP), hipMemcpyHostToDevice));
         if (c->gexec) {
             hipGraphExecDestroy(c->gexec);
             c->gexec = nullptr;
@@ -786,30 +746,11 @@ static int ensure_flags(gmd_ctx *c, int steps) {
     return 0;
 }
 
-// One interaction evaluation (+ integration) as a sequence of launches.
-static void enqueue_phase_step(gmd_ctx *c) {
-    const DP &P = c->P;
-    hipStream_t st = c->stream;
-    if (c->d.nblist_kind == GMD_NBLIST_LINKED_LISTS) {
-        const int gc = std::max(1, std::min((P.ncells + 255) / 256, c->cus * 16));
-        hipLaunchKernelGGL(k_ll_assign, dim3(c->grid_aux), dim3(c->block), 0, st, P);
-        hipLaunchKernelGGL(k_ll_scan, dim3(1), dim3(1024), 1024 * sizeof(int), st, P);
-        hipLaunchKernelGGL(k_ll_scatter, dim3(c->grid_aux), dim3(c->block), 0, st, P);
-        hipLaunchKernelGGL(k_ll_sort, dim3(gc), dim3(256), 0, st, P);
-        const int wpb = c->block / 64;
-        const int gb = std::max(1, std::min((P.ncells + wpb - 1) / wpb, c->cus * 8));
-        hipLaunchKernelGGL(k_ll_build, dim3(gb), dim3(c->block), c->lds_build, st, P);
-    } else {
-        const int gb = std::max(1, (P.N + c->block - 1) / c->block);
-        hipLaunchKernelGGL(k_nsq_build, dim3(gb), dim3(c->block), (size_t)c->block * 16, st, P);
-    }
-    hipLaunchKernelGGL(c->k_stp, dim3(c->grid_step), dim3(c->block), c->lds_step, st, P);
-}
-
 static int run_impl(gmd_ctx *c, float t0, int steps) {
     if (steps < 0) return fail("gmd_run: steps must be non-negative");
     if (ensure_flags(c, steps)) return -1;
     hipStream_t st = c->stream;
+    const DP *dP = c->d_P;
     HIPCHECK(hipMemsetAsync(c->rebuild_flag, 0, (size_t)(steps + 2) * sizeof(int), st));
     if (c->need_rebuild) {
         int one = 1;
@@ -818,24 +759,30 @@ static int run_impl(gmd_ctx *c, float t0, int steps) {
     if (c->d.scalar_save_every > 0)
         HIPCHECK(hipMemsetAsync(c->saver_out, 0, (size_t)c->d.scalar_saves_per_block * 6 * sizeof(float), st));
     hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, c->ctrl, steps, t0, c->buf0, c->lstep);
-    hipLaunchKernelGGL(k_skin_check, dim3(c->grid_aux), dim3(c->block), 0, st, c->P);
+    if (c->d.nblist_kind == GMD_NBLIST_LINKED_LISTS)
+        hipLaunchKernelGGL(k_cells_clear, dim3(std::max(1, std::min((3 * c->P.ncells + 255) / 256, c->cus * 8))),
+                           dim3(256), 0, st, dP);
+    hipLaunchKernelGGL(k_run_setup, dim3(c->grid_aux), dim3(c->block), 0, st, dP);
+    hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(64), 0, st, dP);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev0, st));
     if (c->d.gridsync) {
-        void *args[] = {(void *)&c->P};
+        void *args[] = {(void *)&dP};
         HIPCHECK(hipLaunchCooperativeKernel((const void *)c->k_pers, dim3(c->grid_pers), dim3(c->block), args,
                                             (unsigned)c->lds_pers, st));
     } else if (c->d.use_graph) {
         if (!c->gexec) {
             HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < GRAPH_STEPS; ++k) enqueue_phase_step(c);
+            for (int k = 0; k < GRAPH_STEPS; ++k)
+                hipLaunchKernelGGL(c->k_stp, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
             HIPCHECK(hipStreamEndCapture(st, &c->graph));
             HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
         }
         const int launches = (steps + 1 + GRAPH_STEPS - 1) / GRAPH_STEPS;
         for (int k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(c->gexec, st));
     } else {
-        for (int s = 0; s <= steps; ++s) enqueue_phase_step(c);
+        for (int s = 0; s <= steps; ++s)
+            hipLaunchKernelGGL(c->k_stp, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipEventRecord(c->ev1, st));
@@ -845,6 +792,11 @@ static int run_impl(gmd_ctx *c, float t0, int steps) {
     c->buf0 ^= (steps & 1);
     c->lstep += steps;
     c->need_rebuild = false;
+    Ctrl h;
+    HIPCHECK(hipMemcpy(&h, c->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    if (h.cell_overflow > 0)
+        return fail("linked lists: %d particles fell into one cell, more than the cell capacity %d "
+                    "(use 'N squared' or a larger skin)", h.cell_overflow, c->P.ccap);
     return 0;
 }
 
@@ -954,6 +906,11 @@ static void rdf_thresholds(const float box[3], int bins, std::vector<float> &thr
     }
 }
 
+static size_t rdf_lds_bytes(int nclass, int bins, int nt) {
+    const size_t nh = (size_t)nclass * bins;
+    return ((nh * 4 + 15) & ~(size_t)15) + (size_t)bins * 8 + 256 * 16 + (size_t)nt * nt * 4;
+}
+
 int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
     if (!out) return fail("gmd_rdf_create: null output");
     *out = nullptr;
@@ -964,20 +921,20 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
     r->bins = bins;
     r->nclass = ntypes * (ntypes + 1) / 2;
     const size_t nh = (size_t)r->nclass * bins;
-    const size_t lds = ((nh * 4 + (size_t)(bins + 1) * 4 + 15) & ~(size_t)15) + 256 * 16 + 256 * 4;
-    if (lds > 64 * 1024) {
+    if (rdf_lds_bytes(r->nclass, bins, ntypes) > 64 * 1024) {
         delete r;
-        return fail("gmd_rdf_create: %d type pairs x %d bins do not fit the LDS histogram", r->nclass, bins);
+        return fail("gmd_rdf_create: %d type pairs x %d bins do not fit the LDS histogram", ntypes * (ntypes + 1) / 2,
+                    bins);
     }
     int dev = 0, cus = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const long long T = (N + 255) / 256;
     const long long npairs = T * (T + 1) / 2;
-    r->grid = (int)std::max(1LL, std::min(npairs, (long long)cus * 8));
+    r->grid = (int)std::max(1LL, std::min(npairs, (long long)cus * 2));
     if (hipMalloc(&r->pos, (size_t)N * sizeof(float4)) != hipSuccess ||
         hipMalloc(&r->ptype, (size_t)N * sizeof(int)) != hipSuccess ||
-        hipMalloc(&r->d_thr, (size_t)(bins + 1) * sizeof(float)) != hipSuccess ||
+        hipMalloc(&r->d_thr, (size_t)bins * sizeof(float2)) != hipSuccess ||
         hipMalloc(&r->part, (size_t)r->grid * nh * sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&r->d_out, nh * sizeof(unsigned long long)) != hipSuccess) {
         gmd_rdf_destroy(r);
@@ -1004,7 +961,9 @@ int gmd_rdf_destroy(gmd_rdf *r) {
 static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3], unsigned long long *counts) {
     if (box[0] != r->box[0] || box[1] != r->box[1] || box[2] != r->box[2] || r->thr.empty()) {
         rdf_thresholds(box, r->bins, r->thr);
-        HIPCHECK(hipMemcpy(r->d_thr, r->thr.data(), r->thr.size() * sizeof(float), hipMemcpyHostToDevice));
+        std::vector<float2> t2(r->bins);
+        for (int k = 0; k < r->bins; ++k) t2[k] = make_float2(r->thr[k], r->thr[k + 1]);
+        HIPCHECK(hipMemcpy(r->d_thr, t2.data(), t2.size() * sizeof(float2), hipMemcpyHostToDevice));
         memcpy(r->box, box, sizeof r->box);
     }
     RdfP R;
@@ -1023,16 +982,18 @@ static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const floa
         double bw = ((double)mn / 2.0) / r->bins;
         R.inv_bw = (float)(1.0 / bw);
     }
-    R.pos = pos;
-    R.ptype = d_ptype;
-    R.thr2 = r->d_thr;
-    R.part = r->part;
+    R.pos = (cpos_t)pos;
+    R.ptype = (const GMEM int *)d_ptype;
+    R.thr2 = (const GMEM float2 *)r->d_thr;
+    R.out_thr = r->thr[r->bins];
+    R.part = (GMEM unsigned *)r->part;
     const size_t nh = (size_t)r->nclass * r->bins;
-    const size_t lds = ((nh * 4 + (size_t)(r->bins + 1) * 4 + 15) & ~(size_t)15) + 256 * 16 + 256 * 4;
-    hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), lds, r->stream, R);
+    HIPCHECK(hipMemsetAsync(r->d_out, 0, nh * sizeof(unsigned long long), r->stream));
+    hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), rdf_lds_bytes(r->nclass, r->bins, r->nt), r->stream, R);
     HIPCHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, r->stream, r->part, r->grid,
-                       (int)nh, r->d_out);
+    const int per = 16;
+    hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256), (unsigned)((r->grid + per - 1) / per)),
+                       dim3(256), 0, r->stream, (const unsigned *)r->part, r->grid, (int)nh, per, r->d_out);
     HIPCHECK(hipGetLastError());
     std::vector<unsigned long long> h(nh);
     HIPCHECK(hipMemcpyAsync(h.data(), r->d_out, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));

@@ -30,6 +30,14 @@
 
 #define MAX_TYPES 8
 
+// Pointers into device memory are declared in the global address space in
+// device code, so the compiler emits global_load/store rather than flat ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GMEM __attribute__((address_space(1)))
+#else
+#define GMEM
+#endif
+
 // Control block kept in device memory for the lifetime of a context.
 struct Ctrl {
     float xi, ksum;            // NVT thermostat_state (NVT.py:143)
@@ -42,9 +50,11 @@ struct Ctrl {
     int step;                  // current step (kernel-per-phase path)
     int nsteps;
     int buf0;                  // position buffer holding r at step 0 of the run
+    int rebuild_now;           // rebuild before the interactions of 'step'
     float t0;
+    int cell_overflow;         // largest cell occupancy seen beyond the cell capacity
     long long lstep0;          // Langevin counter of step 0 of the run
-    int pad[4];
+    int pad[2];
 };
 
 // Per-type-pair constants, prepared in LDS at kernel start.
@@ -61,16 +71,17 @@ struct DP {
     float box0, box1, box2;
     float cps2;                // fl32(max_cut + skin)^2 (nblist.py:119)
     float skin2q;              // skin*skin*0.25 (orthorhombic.py:131)
-    float4 *pos0, *pos1, *vel, *frc, *rref, *obeta;
-    float *sc;
-    int *rim;
-    int *nbl, *nbc;
-    // cells
-    int ncx, ncy, ncz, ncells, llcap;
-    int *cell_of, *cell_rank, *cell_count, *cell_start, *cell_members;
-    const int *excl;
+    GMEM float4 *pos0, *pos1, *vel, *frc, *rref, *obeta;
+    GMEM float *sc;
+    GMEM int *rim;
+    GMEM int *nbl, *nbc;
+    // cell grid (linked lists): three buffers, step s reads buffer s%3
+    int ncx, ncy, ncz, ncells, ccap;
+    GMEM int *ccount;          // [3][ncells]
+    GMEM int *cmem;            // [3][ncells*ccap]
+    const GMEM int *excl;
     int excl_cols;
-    const float *raw_params;   // ntypes*ntypes*3
+    const GMEM float *raw_params;   // ntypes*ntypes*3
     int pair_base, cutoff;
     // integrator
     float dt, omega2, alpha;
@@ -80,10 +91,10 @@ struct DP {
     float tpar0, tpar1, tpar2, tpar3;
     int compute_u, compute_w, compute_lap, compute_k, compute_fsq;
     int mom_every, save_every, saves_per_block;
-    Ctrl *ctrl;
-    float *partial;            // [nblocks][GMD_NRED]
-    int *rebuild_flag;         // [nsteps+2]
-    float *saver_out;          // [saves_per_block][6]
+    GMEM Ctrl *ctrl;
+    GMEM float *partial;       // [nblocks][GMD_NRED]
+    GMEM int *rebuild_flag;    // [nsteps+2]
+    GMEM float *saver_out;     // [saves_per_block][6]
     int lds_pp, lds_red, lds_build;   // byte offsets into dynamic LDS
 };
 

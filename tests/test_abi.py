@@ -26,3 +26,20 @@ def test_null_arguments_are_errors():
     assert b"null" in lib.gmd_last_error()
     assert lib.gmd_run(None, ctypes.c_float(0.0), 1) != 0
     assert lib.gmd_destroy(None) == 0
+
+
+def test_every_host_stub_has_a_device_kernel():
+    """Each host launch stub must name a kernel present in the gfx950 code
+    object (an address-space qualifier in a kernel signature mangles the two
+    names differently and the launch would abort)."""
+    import re
+    data = open(_native.LIB_PATH, "rb").read()
+    syms = set(m.decode() for m in re.findall(rb"_Z[0-9A-Za-z_.]+", data))
+    stubs = [s for s in syms if "__device_stub__" in s]
+    assert stubs
+    for stub in stubs:
+        m = re.match(r"_Z(\d+)__device_stub__(.*)$", stub)
+        n = int(m.group(1)) - len("__device_stub__")
+        rest = m.group(2)
+        kernel = f"_Z{n}{rest}"
+        assert kernel + ".kd" in syms, f"{stub}: device kernel {kernel} missing"

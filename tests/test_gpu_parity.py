@@ -100,11 +100,14 @@ def _oracle_for(conf, pp, integrator, **kw):
 
 @pytest.mark.parametrize("driver", ["gridsync", "graph", "plain"])
 def test_nve_trajectory(driver):
-    conf = lj_fcc([5, 5, 5], jitter=0.05, flags={'lapU': True, 'Fsq': True})
+    flags = {'lapU': True, 'Fsq': True}
+    conf = lj_fcc([5, 5, 5], jitter=0.05, flags=flags)
     pp = lj_pair_potential(max_nbs=200)
     o = _oracle_for(conf, pp, "NVE", dt=0.005)
     p = plan(tp=4, gridsync=driver == "gridsync", graph=driver == "graph")
-    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.005), [], num_timeblocks=1, steps_per_timeblock=64,
+    # the ScalarSaver's compute_flags switch on lapU and Fsq (Simulation.py:146-164)
+    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.005), [gp.ScalarSaver(16, flags)], num_timeblocks=1,
+                        steps_per_timeblock=64,
                         compute_plan=p, storage='memory', steps_in_kernel_test=0)
     for _ in sim.run_timeblocks():
         pass
@@ -115,6 +118,7 @@ def test_nve_trajectory(driver):
     np.testing.assert_allclose(conf['U'], o.U, rtol=1e-4, atol=1e-4)
     np.testing.assert_allclose(conf['K'], o.K, rtol=1e-4, atol=1e-4)
     np.testing.assert_allclose(conf['Fsq'], o.fsq, rtol=1e-3, atol=1e-2)
+    np.testing.assert_allclose(conf['lapU'], o.lap, rtol=1e-3, atol=1e-2)
     assert pp.nblist.d_nbflag.copy_to_host()[2] == o.rebuilds
 
 
