@@ -1,0 +1,321 @@
+"""Pinning the oracle (CPU only).
+
+Each restated function is checked against something outside itself:
+known-answer tests of the reference's own test-suite, the rumd3 golden
+snapshots, closed forms, float64 brute force, or a second independent
+restatement (the C oracle).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import md_oracle as orc
+from oracle import c_oracle
+from helpers import GOLDEN, KA_SIG, KA_EPS
+
+F32 = np.float32
+
+
+# --------------------------------------------------------------------------
+# pair functions: tests/test_potential_functions.py:9-10 (exact KATs)
+# --------------------------------------------------------------------------
+def test_pair_function_kats():
+    assert tuple(float(x) for x in orc.lj_12_6(1.0, 2.0, 3.0)) == (5.0, 42.0, 438.0)
+    assert tuple(float(x) for x in orc.lj_sigma_epsilon(1.0, 2.0, 3.0)) == (48384.0, 585216.0, 7635456.0)
+
+
+def test_lj_closed_forms():
+    sig, eps = 1.1, 0.7
+    rmin = 2 ** (1 / 6) * sig
+    u, s, umm = orc.lj_sigma_epsilon(rmin, sig, eps)
+    assert abs(u + eps) < 1e-6 * eps * 10           # u(r_min) = -eps
+    assert abs(s) < 1e-5                             # no force at the minimum
+    assert abs(umm - 72 * eps / (2 ** (1 / 3) * sig ** 2)) < 1e-4
+    # LJ_12_6 with A12 = 4 eps sig^12, A6 = -4 eps sig^6 equals LJ(sigma, eps)
+    r = np.linspace(0.9, 2.4, 50)
+    a = orc.lj_12_6(r, 4 * eps * sig ** 12, -4 * eps * sig ** 6)
+    b = orc.lj_sigma_epsilon(r, sig, eps)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, rtol=2e-5, atol=2e-5)
+    # s = -u'/r: central difference of u in float64
+    r64 = 1.23
+    h = 1e-6
+    up = 4 * eps * ((sig / (r64 + h)) ** 12 - (sig / (r64 + h)) ** 6)
+    um = 4 * eps * ((sig / (r64 - h)) ** 12 - (sig / (r64 - h)) ** 6)
+    s64 = -(up - um) / (2 * h) / r64
+    assert abs(float(orc.lj_sigma_epsilon(r64, sig, eps)[1]) - s64) < 1e-4 * abs(s64)
+
+
+def test_cutoff_wrappers():
+    cut = 2.5
+    u, s, _ = orc.pair_eval("LJ_12_6_sigma_epsilon", "SP", cut, 1.0, 1.0, cut)
+    assert float(u) == 0.0 and float(s) != 0.0            # potential shifted to zero, force jumps
+    u, s, _ = orc.pair_eval("LJ_12_6_sigma_epsilon", "SF", cut, 1.0, 1.0, cut)
+    assert float(u) == 0.0 and float(s) == 0.0            # both go to zero at rc
+    u0, s0, _ = orc.pair_eval("LJ_12_6_sigma_epsilon", None, 1.3, 1.0, 1.0, cut)
+    u1, s1, _ = orc.pair_eval("LJ_12_6_sigma_epsilon", "SP", 1.3, 1.0, 1.0, cut)
+    uc = 4 * (2.5 ** -12 - 2.5 ** -6)
+    assert abs((u0 - u1) - uc) < 1e-6 and s0 == s1
+
+
+# --------------------------------------------------------------------------
+# simulation box: tests/test_simbox.py:5-50
+# --------------------------------------------------------------------------
+def test_simbox_kats():
+    L = np.array([3, 4, 5], F32)
+    ri = np.array([2.5, 3, 4], F32)
+    rj = np.array([0.5, 0.5, 0.5], F32)
+    dr = orc.mic_dr(ri, rj, L)
+    assert np.array_equal(dr, [-1, -1.5, -1.5])
+    assert float(orc.dist_sq(ri, rj, L)) == 5.5
+    r, img = orc.apply_pbc(np.array([4, 5, 6], F32), np.array([0, 1, 2]), L)
+    assert np.array_equal(r, [1, 1, 1]) and np.array_equal(img, [1, 2, 3])
+    r, img = orc.apply_pbc(-np.array([4, 5, 6], F32), np.array([0, 1, 2]), L)
+    assert np.array_equal(r, [-1, -1, -1]) and np.array_equal(img, [-1, 0, 1])
+    rc = np.array([1, 1, 1], F32)
+    assert float(orc.dist_sq(rc, rc + F32(0.75) * L, L)) == 3.125
+    assert not orc.moved_beyond_half_skin(rc, rc + F32(0.5), L, 2.0)
+    assert orc.moved_beyond_half_skin(rc, rc + F32(0.75), L, 2.0)
+
+
+def test_mic_matches_float64_nearest_image():
+    rng = np.random.default_rng(0)
+    L = np.array([7.3, 8.1, 9.7], F32)
+    a = ((rng.random((2000, 3)) - 0.5) * L).astype(F32)
+    b = ((rng.random((2000, 3)) - 0.5) * L).astype(F32)
+    d2 = orc.dist_sq(a, b, L).astype(np.float64)
+    dd = a.astype(np.float64) - b
+    dd -= L * np.rint(dd / L)
+    ref = np.sum(dd * dd, axis=1)
+    np.testing.assert_allclose(d2, ref, rtol=1e-6, atol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# lattice: tests/test_make_lattice.py expected positions (golden fixture)
+# --------------------------------------------------------------------------
+def test_lattice_golden():
+    g = np.load(os.path.join(GOLDEN, "lattice_expected.npz"))
+    pos, box = orc.make_lattice(orc.FCC, [2, 2, 2])
+    np.testing.assert_allclose(pos, g["EXPECTED_FCC_POSITIONS"])
+    np.testing.assert_allclose(box, [2, 2, 2])
+    bcc = {"fractional_coordinates": [[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]], "lattice_constants": [1.0, 1.0, 1.0]}
+    np.testing.assert_allclose(orc.make_lattice(bcc, [2, 2, 2])[0], g["EXPECTED_BCC_POSITIONS"])
+    hexa = {"fractional_coordinates": [[0.0, 0.0], [0.5, 0.5]], "lattice_constants": [1.0, 3 ** 0.5]}
+    np.testing.assert_allclose(orc.make_lattice(hexa, [4, 2])[0], g["EXPECTED_HEXAGONAL_POSITIONS"], atol=1e-7)
+    pos, box = orc.make_lattice(orc.FCC, [8, 8, 8], rho=0.8442)
+    assert abs(len(pos) / np.prod(box) - 0.8442) < 1e-12
+
+
+def test_random_velocities_temperature():
+    v = orc.random_velocities(1000, 3, 1.44, seed=0, m=np.ones(1000, F32))
+    T = np.sum(v.astype(np.float64) ** 2) / (3 * 999)
+    assert abs(T - 1.44) < 1e-5
+    assert np.all(np.abs(np.mean(v, axis=0)) < 1e-6)
+
+
+# --------------------------------------------------------------------------
+# forces: rumd3 golden snapshots (independent code) and float64 brute force
+# --------------------------------------------------------------------------
+def _ka_table():
+    t = np.zeros((2, 2, 3), F32)
+    t[:, :, 0] = KA_SIG
+    t[:, :, 1] = KA_EPS
+    t[:, :, 2] = np.array(KA_SIG) * 2.5
+    return t
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_forces_vs_rumd3_golden(k):
+    d = np.load(os.path.join(GOLDEN, f"ka_rumd3_restart000{k}.npz"))
+    box = d["box"].astype(F32)
+    rows = orc.nblist_nsquared(d["r"], box, 2.5, 0.3)
+    f, U, W, lap = orc.pair_forces(d["r"], d["type"], box, _ka_table(), rows)
+    scale = np.max(np.abs(d["f"]))
+    assert np.max(np.abs(f - d["f"])) / scale < 1e-5
+    assert np.max(np.abs(U - d["pe"])) < 1e-4               # SP cutoff, half of the pair energy
+    assert np.max(np.abs(3 * W - d["vir"])) / np.max(np.abs(d["vir"])) < 1e-5   # vir = D * W
+    f64, U64, W64 = orc.pair_forces_f64(d["r"], d["type"], box, _ka_table())
+    assert np.max(np.abs(f - f64)) / scale < 1e-5
+    # rumd3 mean energies at T=2.0 (energies_mean.dat): pe/N = -4.654 (one snapshot fluctuates ~0.05)
+    assert abs(np.mean(U64) + 4.654) < 0.1
+
+
+def test_sf_forces_vs_float64():
+    pos, box = orc.make_lattice(orc.FCC, [5, 5, 5], rho=0.8442)
+    rng = np.random.default_rng(2)
+    r = (pos + rng.uniform(-0.2, 0.2, pos.shape)).astype(F32)
+    t = np.array([[[1.0, 1.0, 2.5]]], F32)
+    rows = orc.nblist_nsquared(r, box.astype(F32), 2.5, 0.5)
+    for cutoff in ("SP", "SF"):
+        f, U, W, _ = orc.pair_forces(r, np.zeros(len(r)), box.astype(F32), t, rows, cutoff=cutoff)
+        f64, U64, W64 = orc.pair_forces_f64(r, np.zeros(len(r)), box.astype(F32), t, cutoff=cutoff)
+        assert np.max(np.abs(f - f64)) / np.max(np.abs(f64)) < 1e-5
+        # float32 sums of ~60 pair terms of order one: absolute error ~1e-5
+        np.testing.assert_allclose(U, U64, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(W, W64, rtol=1e-4, atol=1e-4)
+    # Newton's third law: total force vanishes
+    assert np.max(np.abs(np.sum(f64, axis=0))) < 1e-9
+
+
+# --------------------------------------------------------------------------
+# neighbour lists
+# --------------------------------------------------------------------------
+def test_nblist_nsquared_vs_float64():
+    pos, box = orc.make_lattice(orc.FCC, [5, 5, 5], rho=0.8442)
+    rng = np.random.default_rng(3)
+    r = (pos + rng.uniform(-0.3, 0.3, pos.shape)).astype(F32)
+    box = box.astype(F32)
+    rows = orc.nblist_nsquared(r, box, 2.5, 0.5)
+    r64 = r.astype(np.float64)
+    thr = 3.0 ** 2
+    for i in range(0, len(r), 7):
+        dd = r64 - r64[i]
+        dd -= box * np.rint(dd / box)
+        d2 = np.sum(dd * dd, axis=1)
+        inside = set(np.nonzero((d2 < thr * (1 - 1e-6)))[0]) - {i}
+        maybe = set(np.nonzero((d2 < thr * (1 + 1e-6)))[0]) - {i}
+        got = set(rows[i].tolist())
+        assert inside <= got <= maybe
+
+
+def test_linked_lists_equal_nsquared():
+    """Reference test_linked_lists_nblist.py:104-106 invariant."""
+    pos, box = orc.make_lattice(orc.FCC, [8, 8, 8], rho=0.8442)
+    np.random.seed(0)
+    r = pos + np.random.uniform(-.3, +.3, pos.shape)
+    r = r[np.random.permutation(len(r))].astype(F32)
+    box = box.astype(F32)
+    a = orc.nblist_nsquared(r, box, 2.5, 0.5)
+    b = orc.nblist_linked_lists(r, box, 2.5, 0.5)
+    c, _ = c_oracle.build_ll(r, box, 2.5, 0.5, 300)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+
+
+def test_niii_half_list_covers_each_pair_once():
+    pos, box = orc.make_lattice(orc.FCC, [4, 4, 4], rho=0.8442)
+    r = pos.astype(F32)
+    box = box.astype(F32)
+    full = orc.nblist_nsquared(r, box, 2.5, 0.5)
+    half = orc.nblist_nsquared(r, box, 2.5, 0.5, niii=True)
+    pairs_full = {(min(i, j), max(i, j)) for i, row in enumerate(full) for j in row}
+    pairs_half = [(min(i, j), max(i, j)) for i, row in enumerate(half) for j in row]
+    assert len(pairs_half) == len(set(pairs_half)) == len(pairs_full)
+
+
+# --------------------------------------------------------------------------
+# integrators: closed-form single steps
+# --------------------------------------------------------------------------
+def test_single_steps_closed_form():
+    box = np.array([10, 10, 10], F32)
+    r = np.array([[0.1, 0.2, 0.3]], F32)
+    v = np.array([[1.0, -2.0, 0.5]], F32)
+    f = np.array([[3.0, 1.0, -2.0]], F32)
+    m = np.array([2.0], F32)
+    img = np.zeros((1, 3), np.int32)
+    dt = 0.01
+    r1, v1, K, fsq, _ = orc.nve_step(r, v, f, m, img, box, dt)
+    v_ref = v + f / m * dt
+    np.testing.assert_allclose(v1, v_ref, rtol=1e-6)
+    np.testing.assert_allclose(r1, r + v_ref * dt, rtol=1e-6)
+    np.testing.assert_allclose(K, 0.5 * 2.0 * np.sum(((v + v_ref) / 2) ** 2), rtol=1e-6)
+    assert float(fsq[0]) == 14.0
+    # NVT with xi = 0 is NVE in velocity; K at the new half step
+    r2, v2, K2, _, _ = orc.nvt_step(r, v, f, m, img, box, dt, 0.0)
+    np.testing.assert_allclose(v2, v_ref, rtol=1e-6)
+    np.testing.assert_allclose(K2, 0.5 * 2.0 * np.sum(v_ref ** 2), rtol=1e-6)
+    # NVT friction: v' = (1 - xi dt/2) v / (1 + xi dt/2) with f = 0
+    xi = 3.0
+    _, v3, _, _, _ = orc.nvt_step(r, v, np.zeros_like(f), m, img, box, dt, xi)
+    np.testing.assert_allclose(v3, v * (1 - xi * dt / 2) / (1 + xi * dt / 2), rtol=1e-6)
+    # Langevin with alpha = 0 is NVE in velocity; K before the update
+    r4, v4, K4, _, _, ob = orc.langevin_step(r, v, f, m, img, box, dt, 0.0, 1.0, np.ones((1, 3)), np.zeros((1, 3)))
+    np.testing.assert_allclose(v4, v_ref, rtol=1e-6)
+    np.testing.assert_allclose(K4, 0.5 * 2.0 * np.sum(v ** 2), rtol=1e-6)
+    # periodic wrap with image update
+    rw, _, _, _, imw = orc.nve_step(np.array([[4.99, 0, 0]], F32), np.array([[10.0, 0, 0]], F32),
+                                    np.zeros((1, 3), F32), m, img, box, dt)
+    assert imw[0, 0] == 1 and abs(rw[0, 0] - (4.99 + 0.1 - 10)) < 1e-5
+
+
+def test_thermostat_update_closed_form():
+    # xi' = xi + dt * omega2 * (2 K / (dof T) - 1)
+    xi = orc.nvt_thermostat_update(0.5, 150.0, 0.004, orc.nvt_omega2(0.2), 297, 0.8)
+    ref = 0.5 + 0.004 * (4 * np.pi ** 2 / 0.04) * (300.0 / (297 * 0.8) - 1)
+    assert abs(xi - ref) < 1e-5
+
+
+def test_nve_energy_conservation():
+    """Reference tests/test_LJ_nxyz.py:107: var(E)/N < 1e-4 (NVE, SF cutoff)."""
+    pos, box = orc.make_lattice(orc.FCC, [4, 4, 4], rho=0.8442)
+    v = orc.random_velocities(len(pos), 3, 1.44, seed=0, m=np.ones(len(pos), F32))
+    t = np.array([[[1.0, 1.0, 2.5]]], F32)
+    o = orc.OracleSystem(pos.astype(F32), v, np.ones(len(pos), F32), np.zeros(len(pos), np.int32), box, t, 2.5, 0.5,
+                         cutoff="SF", integrator="NVE", dt=0.005, save_every=10)
+    o.run_block(600)
+    E = np.array([s[0] + s[3] for s in o.saved])
+    assert np.var(E) / len(pos) < 1e-4
+
+
+def test_momentum_reset():
+    rng = np.random.default_rng(4)
+    v = rng.normal(size=(500, 3)).astype(F32)
+    m = rng.uniform(0.5, 2.0, 500).astype(F32)
+    v2 = orc.momentum_reset(v, m)
+    p = np.sum(m[:, None].astype(np.float64) * v2, axis=0)
+    assert np.all(np.abs(p) < 1e-4)
+    # only the centre-of-mass velocity changes
+    np.testing.assert_allclose(v - v2, np.broadcast_to((v - v2)[0], v.shape), atol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# Philox-4x32-10: known-answer vectors of the Random123 distribution
+# --------------------------------------------------------------------------
+def test_philox_kat():
+    out = orc.philox4x32_10([0], [0], [0], [0], 0, 0)
+    assert [int(x[0]) for x in out] == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    m = 0xFFFFFFFF
+    out = orc.philox4x32_10([m], [m], [m], [m], m, m)
+    assert [int(x[0]) for x in out] == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    out = orc.philox4x32_10([0x243f6a88], [0x85a308d3], [0x13198a2e], [0x03707344], 0xa4093822, 0x299f31d0)
+    assert [int(x[0]) for x in out] == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_langevin_normals_statistics():
+    z = orc.langevin_normals(213, 5, np.arange(200_000))
+    assert abs(np.mean(z)) < 0.01 and abs(np.std(z) - 1) < 0.01
+    assert abs(np.mean(z[:, 0] * z[:, 1])) < 0.01
+
+
+# --------------------------------------------------------------------------
+# radial distribution
+# --------------------------------------------------------------------------
+def test_rdf_counts_numpy_equals_c():
+    d = np.load(os.path.join(GOLDEN, "ka_rumd3_restart0000.npz"))
+    r = d["r"][::3]
+    pt = d["type"][::3]
+    box = d["box"].astype(F32)
+    assert 0 < np.sum(pt) < len(pt)
+    a = orc.rdf_counts(r, pt, box, 300, ntypes=2)
+    b = c_oracle.rdf_counts(r, pt, box, 300, 2)
+    assert np.array_equal(a, b)
+    # every ordered pair inside min(L)/2 counted once
+    r64 = r.astype(np.float64)
+    n_in = 0
+    for i in range(len(r)):
+        dd = r64 - r64[i]
+        dd -= box * np.rint(dd / box)
+        dist = np.sqrt(np.sum(dd * dd, axis=1))
+        n_in += int(np.sum(dist < box.min() / 2)) - 1
+    assert abs(int(a.sum()) - n_in) <= 2
+    assert np.array_equal(a[0, 1], a[1, 0])
+
+
+def test_rdf_bin_edges_match_float64_rule():
+    box = np.array([14.938016, 14.938016, 14.938016], F32)
+    half, bw = orc.rdf_bin_width(box, 1000)
+    d = np.array([0.0, bw * 0.5, bw * 1.0, bw * 999.5, half * (1 - 1e-7), half], np.float64).astype(F32)
+    b = orc.rdf_bin_of(d, box, 1000)
+    assert b[0] == 0 and b[1] == 0 and b[3] == 999 and b[5] == -1
+    assert b[2] in (0, 1)
