@@ -77,7 +77,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(independent replicas shard one set per GPU; see DESIGN.md)", file=sys.stderr)
+              "(independent replicas would shard one set per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import gamdpy_amd as gp

@@ -28,6 +28,7 @@ from . import integrators
 from .integrators import Integrator, NVE, NVT, NVT_Langevin
 from .runtime_actions import RuntimeAction, MomentumReset, ScalarSaver
 from .simulation import Simulation, get_default_compute_plan
+from .batch import BatchedSimulation, Replica
 from .calculators import CalculatorRadialDistribution
 from .evaluator import Evaluator
 from .misc import make_function_constant, make_function_ramp, make_function_sin, extract_scalars

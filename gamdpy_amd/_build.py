@@ -10,7 +10,7 @@ import sys
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
 SOURCES = [os.path.join(_HERE, "csrc", "gmd_core.hip")]
-DEPS = [os.path.join(_HERE, "csrc", f) for f in ("gmd_device.h", "gmd_kernels.h")] + \
+DEPS = [os.path.join(_HERE, "csrc", f) for f in ("gmd_device.h", "gmd_kernels.h", "gmd_adjoint.h")] + \
        [os.path.join(ROOT, "include", "gamdpy_amd.h")]
 OUT = os.path.join(_HERE, "libgamdpy_amd.so")
 ARCH = os.environ.get("GMD_OFFLOAD_ARCH", "gfx950")

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgamdpy_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gamdpy_amd.h")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # enums of gamdpy_amd.h
 PAIR_BASE = {"LJ_12_6_sigma_epsilon": 0, "LJ_12_6": 1}
@@ -71,6 +71,15 @@ class GmdDeviceInfo(ctypes.Structure):
                 ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 64)]
 
 
+class GmdAdjDesc(ctypes.Structure):
+    """Mirror of ``gmd_adj_desc`` (differentiable NVE trajectory)."""
+    _fields_ = [("N", ctypes.c_int), ("ntypes", ctypes.c_int), ("box", ctypes.c_float * 3), ("dt", ctypes.c_float),
+                ("nsteps", ctypes.c_int), ("ckpt_every", ctypes.c_int), ("rdf_every", ctypes.c_int),
+                ("rdf_bins", ctypes.c_int), ("rdf_rmax", ctypes.c_float), ("rdf_width", ctypes.c_float)]
+
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
 _SIGNATURES = {
     "gmd_abi_version": (ctypes.c_int, []),
     "gmd_desc_size": (ctypes.c_int, []),
@@ -93,7 +102,22 @@ _SIGNATURES = {
     "gmd_rdf_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "gmd_rdf_compute_ctx": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int_p, c_u64_p]),
     "gmd_rdf_compute_host": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_int_p, c_float_p, c_u64_p]),
+    "gmd_batch_create": (ctypes.c_int, [ctypes.POINTER(GmdDesc), ctypes.c_int, c_int_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "gmd_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_batch_size": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
+    "gmd_batch_context": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "gmd_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]),
+    "gmd_batch_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "gmd_adj_create": (ctypes.c_int, [ctypes.POINTER(GmdAdjDesc), c_int_p, c_float_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "gmd_adj_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_adj_forward": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_double_p, c_float_p,
+                                       c_float_p]),
+    "gmd_adj_backward": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, c_float_p, c_float_p]),
+    "gmd_adj_last_ms": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p]),
 }
+
+BATCH_DRIVER = {"persistent": 0, "graph": 1}
 
 _lib = None
 _lock = threading.Lock()
@@ -164,36 +188,61 @@ def device_info():
     return _device_info
 
 
+def fill_desc(d, desc_fields, params, exclusions=None):
+    """Fill a GmdDesc from a dict of field values; returns the arrays that the
+    descriptor points to (keep them alive while the descriptor is used)."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    excl = None if exclusions is None else np.ascontiguousarray(exclusions, dtype=np.int32)
+    d.abi_version = ABI_VERSION
+    for k, v in desc_fields.items():
+        if k == "box":
+            d.box = (ctypes.c_float * 3)(*[float(x) for x in v])
+        elif k == "tparams":
+            d.tparams = (ctypes.c_float * 4)(*[float(x) for x in v])
+        else:
+            setattr(d, k, v)
+    d.params = fptr(params)
+    if excl is not None:
+        d.exclusions = iptr(excl)
+        d.exclusions_cols = excl.shape[1]
+    return params, excl
+
+
 class Context:
-    """Owns one gmd_ctx: the device state of one Simulation (or Evaluator)."""
+    """Owns one gmd_ctx: the device state of one Simulation (or Evaluator).
+    A replica of a Batch is a borrowed context (``Context.borrowed``): the
+    batch owns it and runs it."""
 
     def __init__(self, desc_fields, params, exclusions=None):
         self.lib = load()
-        self._params = np.ascontiguousarray(params, dtype=np.float32)
-        self._excl = None if exclusions is None else np.ascontiguousarray(exclusions, dtype=np.int32)
         d = GmdDesc()
-        d.abi_version = ABI_VERSION
-        for k, v in desc_fields.items():
-            if k == "box":
-                d.box = (ctypes.c_float * 3)(*[float(x) for x in v])
-            elif k == "tparams":
-                d.tparams = (ctypes.c_float * 4)(*[float(x) for x in v])
-            else:
-                setattr(d, k, v)
-        d.params = fptr(self._params)
-        if self._excl is not None:
-            d.exclusions = iptr(self._excl)
-            d.exclusions_cols = self._excl.shape[1]
+        self._params, self._excl = fill_desc(d, desc_fields, params, exclusions)
+        self._init_fields(d)
+        self._owner = None
+        h = ctypes.c_void_p()
+        check(self.lib.gmd_create(ctypes.byref(d), ctypes.byref(h)), "gmd_create")
+        self.h = h
+
+    def _init_fields(self, d):
         self.desc = d
         self.N = d.N
         self.nscalars = d.nscalars
         self.max_nbs = d.max_nbs
         self.saves_per_block = d.scalar_saves_per_block
-        h = ctypes.c_void_p()
-        check(self.lib.gmd_create(ctypes.byref(d), ctypes.byref(h)), "gmd_create")
-        self.h = h
+
+    @classmethod
+    def borrowed(cls, owner, handle, desc):
+        self = cls.__new__(cls)
+        self.lib = load()
+        self._init_fields(desc)
+        self._owner = owner
+        self.h = handle
+        return self
 
     def close(self):
+        if self._owner is not None:
+            self.h = None
+            return
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.gmd_destroy(self.h)
             self.h = None
@@ -259,6 +308,60 @@ class Context:
         a = np.zeros((self.saves_per_block, len(SAVE_COLUMNS)), np.float32)
         check(self.lib.gmd_read_scalar_output(self.h, fptr(a)), "gmd_read_scalar_output")
         return a
+
+
+class Batch:
+    """Owns one gmd_batch: independent replicas advanced by one launch per run
+    (driver 'persistent') or one launch per step in a hipGraph ('graph').
+    ``replicas`` is a list of (desc_fields, params, exclusions)."""
+
+    def __init__(self, replicas, blocks=None, driver="persistent"):
+        self.lib = load()
+        n = len(replicas)
+        descs = (GmdDesc * n)()
+        self._keep = []
+        for k, (fields, params, excl) in enumerate(replicas):
+            self._keep.append(fill_desc(descs[k], fields, params, excl))
+        self._descs = descs
+        blk = None if blocks is None else np.ascontiguousarray(blocks, dtype=np.int32)
+        if blk is not None and blk.shape != (n,):
+            raise ValueError(f"blocks must hold one count per replica ({n})")
+        h = ctypes.c_void_p()
+        check(self.lib.gmd_batch_create(descs, n, iptr(blk), BATCH_DRIVER[driver], ctypes.byref(h)),
+              "gmd_batch_create")
+        self.h = h
+        self.driver = driver
+        self.contexts = []
+        for k in range(n):
+            c = ctypes.c_void_p()
+            check(self.lib.gmd_batch_context(self.h, k, ctypes.byref(c)), "gmd_batch_context")
+            self.contexts.append(Context.borrowed(self, c, descs[k]))
+
+    def size(self):
+        n, g = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.gmd_batch_size(self.h, ctypes.byref(n), ctypes.byref(g)), "gmd_batch_size")
+        return n.value, g.value
+
+    def run(self, time_zero, steps):
+        check(self.lib.gmd_batch_run(self.h, ctypes.c_float(np.float32(time_zero)), int(steps)), "gmd_batch_run")
+
+    def last_run_ms(self):
+        x = ctypes.c_float()
+        check(self.lib.gmd_batch_last_run_ms(self.h, ctypes.byref(x)), "gmd_batch_last_run_ms")
+        return float(x.value)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            for c in self.contexts:
+                c.h = None
+            self.lib.gmd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class RdfHandle:

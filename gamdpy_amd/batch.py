@@ -1,0 +1,248 @@
+"""Batches of independent systems advanced together on one GPU.
+
+The reference runs one system per ``Simulation``; a parameter sweep or an
+ensemble is a Python loop over simulations, each launch filling only a
+fraction of the device.  ``BatchedSimulation`` takes the same building
+blocks per replica (Configuration, PairPotential, integrator, runtime
+actions) and advances all replicas with one kernel launch per run
+(``driver='persistent'``: the blocks of each replica meet at their own
+barrier each step, replicas never wait for each other) or one launch per
+step for all replicas (``driver='graph'``).
+
+Replicas may differ in N, box, density, temperature, potential parameters,
+neighbour-list kind and runtime actions; they must share the pair function,
+cutoff wrapper, integrator class, number of particle types and lanes per
+particle (one kernel instance runs them all).
+
+``autotune()`` chooses the driver, lanes per particle and the number of
+blocks each replica gets (blocks in proportion to the replica's work, under a
+total that keeps the persistent grid co-resident) by timing short runs.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import _native
+from .interactions import PairPotential
+from .output import open_output
+from .simulation import build_descriptor, resolve_compute_flags, _ll_cells_ok
+
+
+class Replica:
+    """One system of a batch: the arguments a Simulation would take."""
+
+    def __init__(self, configuration, interactions, integrator, runtime_actions=()):
+        self.configuration = configuration
+        self.interaction = interactions[0] if isinstance(interactions, list) else interactions
+        if not isinstance(self.interaction, PairPotential):
+            raise NotImplementedError("a replica runs exactly one PairPotential")
+        self.integrator = integrator
+        self.runtime_actions = list(runtime_actions)
+        self.compute_flags = resolve_compute_flags(configuration, self.runtime_actions)
+        self.output = None
+        self.ctx = None
+
+    def list_length(self, skin):
+        """Mean full-list length rho (4 pi / 3) (rc + skin)^3 of this replica."""
+        conf = self.configuration
+        rho = conf.N / float(np.prod(conf.simbox.lengths))
+        rc = float(self.interaction.param_table()[1])
+        return rho * 4.0 * math.pi / 3.0 * (rc + skin) ** 3
+
+    def work(self, skin):
+        """Relative cost of one step: list entries plus per-particle work."""
+        return self.configuration.N * (self.list_length(skin) + 20.0)
+
+
+def replica_plan(replica, tp, skin):
+    """Per-replica compute plan: linked lists where the box admits them and N
+    is large enough to gain from them, else N squared."""
+    conf = replica.configuration
+    rc = float(replica.interaction.param_table()[1])
+    ll = conf.N > 6000 and _ll_cells_ok(conf, rc, skin)
+    return {'pb': max(1, 256 // tp), 'tp': tp, 'skin': skin, 'UtilizeNIII': False, 'gridsync': False,
+            'nblist': 'linked lists' if ll else 'N squared', 'graph': False}
+
+
+def assign_blocks(replicas, tp, skin, total, block=256):
+    """Blocks per replica in proportion to its work, at least one each, at most
+    one particle slot per thread, summing to about ``total``."""
+    w = np.array([r.work(skin) for r in replicas], np.float64)
+    cap = np.array([max(1, math.ceil(r.configuration.N * tp / block)) for r in replicas])
+    n = np.maximum(1, np.floor(total * w / w.sum())).astype(int)
+    return np.minimum(n, cap)
+
+
+class BatchedSimulation:
+    """Advance ``replicas`` (a list of Replica) by timeblocks.
+
+    Mirrors Simulation's interface (``run_timeblocks``, ``run``,
+    ``summary``); outputs are per replica (``outputs[k]``, the object a
+    Simulation would expose as ``sim.output``).
+    """
+
+    def __init__(self, replicas, num_timeblocks, steps_per_timeblock, tp=4, skin=0.5, driver='persistent',
+                 blocks=None, storage='memory', timing=True):
+        if not replicas:
+            raise ValueError("a batch needs at least one replica")
+        self.replicas = list(replicas)
+        self.num_blocks = num_timeblocks
+        self.steps_per_block = steps_per_timeblock
+        dts = {float(np.float32(r.integrator.dt)) for r in self.replicas}
+        if len(dts) != 1:
+            raise ValueError("replicas of one batch share the timestep dt")
+        self.dt = dts.pop()
+        self.tp, self.skin, self.driver = tp, skin, driver
+        self.timing = timing
+        self.outputs = []
+        for r in self.replicas:
+            out = open_output(storage)
+            out.attrs['dt'] = self.dt
+            out.attrs['simbox_initial'] = r.configuration.simbox.lengths
+            out.create_dataset("ptype", data=r.configuration.ptype, shape=(r.configuration.N,), dtype=np.int32)
+            for ra in r.runtime_actions:
+                ra.setup(configuration=r.configuration, num_timeblocks=num_timeblocks,
+                         steps_per_timeblock=steps_per_timeblock, output=out)
+            r.output = out
+            self.outputs.append(out)
+        self.batch = None
+        self.blocks = None
+        self._build(tp, skin, driver, blocks)
+
+    # ------------------------------------------------------------------
+    def _build(self, tp, skin, driver, blocks):
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+        specs = []
+        for r in self.replicas:
+            plan = replica_plan(r, tp, skin)
+            specs.append(build_descriptor(r.configuration, r.interaction, r.integrator, r.runtime_actions, plan,
+                                          r.compute_flags))
+        if blocks is None:
+            blocks = self.default_blocks(tp, skin, driver)
+        self.batch = _native.Batch(specs, blocks=blocks, driver=driver)
+        self.blocks = None if blocks is None else np.asarray(blocks)
+        self.tp, self.skin, self.driver = tp, skin, driver
+        for r, ctx in zip(self.replicas, self.batch.contexts):
+            r.ctx = ctx
+            r.configuration.attach(ctx)
+            r.interaction.nblist.bind(ctx)
+            r.configuration.copy_to_device()
+            r.integrator.upload_state(ctx)
+
+    def default_blocks(self, tp, skin, driver):
+        """Persistent: blocks in proportion to work within two blocks per compute
+        unit (co-resident at 256 threads); graph: one particle slot per thread."""
+        if driver == 'graph':
+            return None
+        try:
+            cus = _native.device_info()["compute_units"]
+        except Exception:
+            cus = 256
+        total = max(len(self.replicas), 2 * cus)
+        return assign_blocks(self.replicas, tp, skin, total)
+
+    # ------------------------------------------------------------------
+    def integrate_self(self, time_zero, steps):
+        self.batch.run(np.float32(time_zero), steps)
+
+    def run_timeblocks(self, num_timeblocks=-1):
+        if num_timeblocks == -1:
+            num_timeblocks = self.num_blocks
+        self.last_num_blocks = num_timeblocks
+        for r in self.replicas:
+            r.configuration.copy_to_device()
+            r.integrator.upload_state(r.ctx)
+        block_times = []
+        t_start = time.perf_counter()
+        for block in range(num_timeblocks):
+            self.current_block = block
+            for r in self.replicas:
+                for ra in r.runtime_actions:
+                    ra.initialize_before_timeblock()
+            self.integrate_self(np.float32(block * self.steps_per_block * self.dt), self.steps_per_block)
+            block_times.append(self.batch.last_run_ms())
+            for r in self.replicas:
+                r.configuration.copy_to_host()
+                r.integrator.download_state(r.ctx)
+                r.interaction.check_datastructure_validity()
+                for ra in r.runtime_actions:
+                    ra.update_at_end_of_timeblock(block, r.output, ctx=r.ctx)
+            yield block
+        self.timing_total_ms = (time.perf_counter() - t_start) * 1000.0
+        self.timing_blocks = np.array(block_times)
+
+    def run(self, verbose=True):
+        for _ in self.run_timeblocks():
+            pass
+        if verbose:
+            print(self.summary())
+
+    def particle_steps(self, blocks=None):
+        nb = self.last_num_blocks if blocks is None else blocks
+        return sum(r.configuration.N for r in self.replicas) * nb * self.steps_per_block
+
+    def summary(self):
+        n = len(self.replicas)
+        ntot = sum(r.configuration.N for r in self.replicas)
+        st = f'Replicas : {n} ({ntot} particles, driver {self.driver}, tp {self.tp})\n'
+        st += f'Steps : {self.last_num_blocks} * {self.steps_per_block}\n'
+        t = np.sum(self.timing_blocks) / 1000.0
+        st += f'Simulation time: {t:.3f} s ( {self.particle_steps() / t:.3e} particle-steps/s )\n'
+        return st
+
+    # ------------------------------------------------------------------
+    def autotune(self, steps=200, tps=(1, 2, 4, 8), drivers=('persistent', 'graph'), fractions=(1.0, 2.0, 4.0),
+                 verbose=False):
+        """Time short runs of each (driver, tp, block budget) and keep the
+        fastest.  The persistent budget is fractions x compute units blocks;
+        plans the device cannot hold co-resident are skipped.  The replicas'
+        states are restored after every trial."""
+        saved = [(r.configuration['r'].copy(), r.configuration['v'].copy(), r.configuration.r_im.copy(),
+                  getattr(r.integrator, 'thermostat_state', None)) for r in self.replicas]
+        saved = [(a, b, c, None if d is None else d.copy()) for a, b, c, d in saved]
+        try:
+            cus = _native.device_info()["compute_units"]
+        except Exception:
+            cus = 256
+        trials = []
+        for driver in drivers:
+            for tp in tps:
+                budgets = [None] if driver == 'graph' else [int(f * cus) for f in fractions]
+                for total in budgets:
+                    blocks = None if total is None else assign_blocks(self.replicas, tp, self.skin,
+                                                                       max(total, len(self.replicas)))
+                    trials.append((driver, tp, blocks))
+        best = (float('inf'), None)
+        for driver, tp, blocks in trials:
+            self._restore(saved)
+            try:
+                self._build(tp, self.skin, driver, blocks)
+                self.integrate_self(0.0, 2)
+                self.integrate_self(0.0, steps)
+                ms = self.batch.last_run_ms()
+            except _native.NativeError as e:
+                if verbose:
+                    print(f'{driver} tp={tp}: {e}')
+                continue
+            if verbose:
+                tot = None if blocks is None else int(np.sum(blocks))
+                print(f'{driver:10s} tp={tp} blocks={tot}: {ms / steps * 1e3:.1f} us/step')
+            if ms < best[0]:
+                best = (ms, (driver, tp, blocks))
+        if best[1] is None:
+            raise RuntimeError("autotune: no plan ran")
+        self._restore(saved)
+        driver, tp, blocks = best[1]
+        self._build(tp, self.skin, driver, blocks)
+        return {'driver': driver, 'tp': tp, 'blocks': None if blocks is None else np.asarray(blocks)}
+
+    def _restore(self, saved):
+        for r, (pos, vel, im, th) in zip(self.replicas, saved):
+            r.configuration['r'] = pos
+            r.configuration['v'] = vel
+            r.configuration.r_im = im
+            if th is not None:
+                r.integrator.thermostat_state = th.copy()

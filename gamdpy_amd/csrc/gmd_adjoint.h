@@ -1,0 +1,615 @@
+// gmd_adjoint.h -- differentiable NVE trajectory: float32 forward with
+// checkpoints, hand-written adjoint (force Jacobian-vector products and
+// gradients with respect to the per-type LJ sigma / epsilon), and a
+// Gaussian-smoothed partial radial distribution as the observable.
+//
+// Included once into gmd_core.hip (shares its error channel and helpers).
+//
+// Model: LJ_12_6_sigma_epsilon with the shifted-force cutoff as gamdpy
+// defines it (apply_shifted_force_cutoff.py:29-37: s(r) - s(rc)), at a cut
+// held fixed while sigma and epsilon vary; leap-frog NVE as NVE.py:59-105
+// (v += f/m dt; r += v dt; wrap).
+//
+// Forward, for n = 0 .. S-1 (state x_n = (r_n, v_{n-1/2})):
+//   F_n = F(r_n),  v_{n+1/2} = v_{n-1/2} + F_n/m dt,  r_{n+1} = r_n + v_{n+1/2} dt
+// Observable: H[c][k] = sum over frames n (n % rdf_every == 0) and unordered
+// pairs of class c of K(d_ij - r_k), K(x) = max(0, exp(-x^2/2w^2) - exp(-12.5))
+// for |x| < 5w, r_k = (k + 1/2) rmax / bins.
+// Adjoint (lambda_r = dL/dr_n, lambda_v = dL/dv_{n-1/2}), n = S-1 .. 0:
+//   lambda_v <- lambda_v + dt lambda_r(n+1);  w = dt/m lambda_v
+//   lambda_r(n) = lambda_r(n+1) + J(r_n) w + dL/dr_n|frame
+//   dL/dtheta  += w . dF(r_n)/dtheta
+// with J = dF/dr symmetric: (J w)_i = sum_j A_ij (w_i - w_j) + B_ij d_ij (d_ij . (w_i - w_j)),
+// A = s(r), B = (ds/dr)/r, d_ij = r_i - r_j (minimum image).
+#pragma once
+
+#define ADJ_MAX_TYPES 3
+#define ADJ_MAX_CLS 6            // ADJ_MAX_TYPES (ADJ_MAX_TYPES + 1) / 2
+#define ADJ_BI 128               // particles i per block
+#define ADJ_JC 256               // particles j per block (staged in LDS)
+#define ADJ_MAX_BINS 512
+
+struct AdjPairC {                // one ordered type pair
+    float sig, sig2, eps, eps24; // sigma, sigma^2, epsilon, 24 epsilon
+    float cut2, sc, dsc_dsig;    // cut^2, s(rc), ds(rc)/dsigma at fixed rc
+    int cls;                     // unordered class index
+};
+
+struct AdjK {
+    int N, nt, ncls, nchunk;
+    float L0, L1, L2;
+    float dt;
+    AdjPairC pc[ADJ_MAX_TYPES * ADJ_MAX_TYPES];
+    // observable
+    int bins;
+    float rmax, dr, wid, inv_w2, kfloor, rcut2;   // rcut = rmax + 5 w
+};
+
+__device__ __forceinline__ int adj_type(float4 r) { return __float_as_int(r.w); }
+
+// s(r) - s(rc) of the LJ pair function, and the pieces the adjoint needs
+struct AdjPairEval {
+    float s, B, ds_deps, ds_dsig;
+};
+
+__device__ __forceinline__ AdjPairEval adj_pair(const AdjPairC &q, float d2) {
+    const float inv_r2 = 1.0f / d2;
+    const float x = q.sig2 * inv_r2;
+    const float x3 = x * x * x;
+    const float x6 = x3 * x3;
+    AdjPairEval e;
+    const float s_lj = q.eps24 * (2.0f * x6 - x3) * inv_r2;
+    e.s = s_lj - q.sc;
+    e.B = q.eps24 * (8.0f * x3 - 28.0f * x6) * inv_r2 * inv_r2;
+    e.ds_deps = e.s / q.eps;
+    e.ds_dsig = (q.eps24 / q.sig) * (24.0f * x6 - 6.0f * x3) * inv_r2 - q.dsc_dsig;
+    return e;
+}
+
+// Pair sums over one chunk of ADJ_JC particles j for ADJ_BI particles i.
+//   MODE 0: force      part[chunk][i] = sum_j s d_ij
+//   MODE 1: J w        part[chunk][i] = sum_j A (w_i - w_j) + B d (d.(w_i - w_j)),
+//                      pgrad[block][2c + {0,1}] = sum_i w_i . d ds/d{sigma,epsilon}
+template <int MODE>
+__global__ void __launch_bounds__(ADJ_BI) k_adj_pairs(AdjK K, const float4 *pos, const float4 *w, float4 *part,
+                                                      double *pgrad) {
+    __shared__ float4 sp[ADJ_JC];
+    __shared__ float4 sw[MODE == 1 ? ADJ_JC : 1];
+    __shared__ float red[ADJ_BI / 64][2 * ADJ_MAX_CLS];
+    const int i = blockIdx.x * ADJ_BI + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * ADJ_JC;
+    const int jn = min(ADJ_JC, K.N - j0);
+    for (int t = threadIdx.x; t < jn; t += ADJ_BI) {
+        sp[t] = pos[j0 + t];
+        if (MODE == 1) sw[t] = w[j0 + t];
+    }
+    __syncthreads();
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float g[2 * ADJ_MAX_CLS];
+#pragma unroll
+    for (int c = 0; c < 2 * ADJ_MAX_CLS; ++c) g[c] = 0.f;
+    if (i < K.N) {
+        const float4 ri = pos[i];
+        const int ti = adj_type(ri);
+        float4 wi = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MODE == 1) wi = w[i];
+        for (int t = 0; t < jn; ++t) {
+            if (j0 + t == i) continue;
+            const float4 rj = sp[t];
+            float dx, dy, dz;
+            const float d2 = dist_sq_mic(ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, K.L0, K.L1, K.L2, dx, dy, dz);
+            const AdjPairC &q = K.pc[ti * K.nt + adj_type(rj)];
+            if (d2 >= q.cut2) continue;
+            const AdjPairEval e = adj_pair(q, d2);
+            if (MODE == 0) {
+                ax += e.s * dx;
+                ay += e.s * dy;
+                az += e.s * dz;
+            } else {
+                const float4 wj = sw[t];
+                const float ux = wi.x - wj.x, uy = wi.y - wj.y, uz = wi.z - wj.z;
+                const float du = dx * ux + dy * uy + dz * uz;
+                ax += e.s * ux + e.B * dx * du;
+                ay += e.s * uy + e.B * dy * du;
+                az += e.s * uz + e.B * dz * du;
+                const float wd = wi.x * dx + wi.y * dy + wi.z * dz;
+#pragma unroll
+                for (int c = 0; c < ADJ_MAX_CLS; ++c)
+                    if (c == q.cls) {
+                        g[2 * c] += wd * e.ds_dsig;
+                        g[2 * c + 1] += wd * e.ds_deps;
+                    }
+            }
+        }
+        part[(size_t)chunk * K.N + i] = make_float4(ax, ay, az, 0.f);
+    }
+    if (MODE == 1) {
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int c = 0; c < 2 * ADJ_MAX_CLS; ++c) {
+            const float v = wave_sum_f(g[c]);
+            if (lane == 0) red[wid][c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * ADJ_MAX_CLS) {
+            double s = 0.0;
+            for (int k = 0; k < ADJ_BI / 64; ++k) s += (double)red[k][threadIdx.x];
+            pgrad[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (2 * ADJ_MAX_CLS) + threadIdx.x] = s;
+        }
+    }
+}
+
+// forward leap-frog step of particle i from the chunk partial forces;
+// rec (optional) receives the new position
+__global__ void __launch_bounds__(256) k_adj_step(AdjK K, float4 *pos, float4 *vel, const float4 *part,
+                                                  float4 *rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K.N) return;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int c = 0; c < K.nchunk; ++c) {
+        const float4 p = part[(size_t)c * K.N + i];
+        fx += p.x;
+        fy += p.y;
+        fz += p.z;
+    }
+    float4 v = vel[i], r = pos[i];
+    {
+#pragma clang fp contract(off)
+        const float inv_m = 1.0f / v.w;
+        v.x = v.x + fx * inv_m * K.dt;
+        v.y = v.y + fy * inv_m * K.dt;
+        v.z = v.z + fz * inv_m * K.dt;
+        r.x = r.x + v.x * K.dt;
+        r.y = r.y + v.y * K.dt;
+        r.z = r.z + v.z * K.dt;
+    }
+    int im = 0;
+    wrap_pbc(r.x, im, K.L0);
+    wrap_pbc(r.y, im, K.L1);
+    wrap_pbc(r.z, im, K.L2);
+    vel[i] = v;
+    pos[i] = r;
+    if (rec) rec[i] = r;
+}
+
+// smoothed pair histogram of one frame: unordered pairs (i < j), one block
+// partial per (i-tile, j-chunk) written to hpart[block][ncls * bins]
+__global__ void __launch_bounds__(ADJ_BI) k_adj_rdf(AdjK K, const float4 *pos, float *hpart) {
+    __shared__ float4 sp[ADJ_JC];
+    __shared__ float h[ADJ_MAX_CLS * ADJ_MAX_BINS];
+    const int nh = K.ncls * K.bins;
+    for (int t = threadIdx.x; t < nh; t += ADJ_BI) h[t] = 0.f;
+    const int i = blockIdx.x * ADJ_BI + threadIdx.x;
+    const int j0 = blockIdx.y * ADJ_JC;
+    const int jn = min(ADJ_JC, K.N - j0);
+    for (int t = threadIdx.x; t < jn; t += ADJ_BI) sp[t] = pos[j0 + t];
+    __syncthreads();
+    if (i < K.N) {
+        const float4 ri = pos[i];
+        const int ti = adj_type(ri);
+        for (int t = 0; t < jn; ++t) {
+            if (j0 + t <= i) continue;
+            const float4 rj = sp[t];
+            float dx, dy, dz;
+            const float d2 = dist_sq_mic(ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, K.L0, K.L1, K.L2, dx, dy, dz);
+            if (d2 >= K.rcut2) continue;
+            const float d = sqrtf(d2);
+            const int c = K.pc[ti * K.nt + adj_type(rj)].cls;
+            const int k0 = max(0, (int)floorf((d - 5.0f * K.wid) / K.dr - 0.5f));
+            const int k1 = min(K.bins - 1, (int)ceilf((d + 5.0f * K.wid) / K.dr - 0.5f));
+            for (int k = k0; k <= k1; ++k) {
+                const float x = d - ((float)k + 0.5f) * K.dr;
+                const float kv = __expf(-0.5f * x * x * K.inv_w2) - K.kfloor;
+                if (kv > 0.f) atomicAdd(&h[c * K.bins + k], kv);
+            }
+        }
+    }
+    __syncthreads();
+    float *dst = hpart + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * nh;
+    for (int t = threadIdx.x; t < nh; t += ADJ_BI) dst[t] = h[t];
+}
+
+// H[k] += sum over block partials (fixed order, float64)
+__global__ void k_adj_rdf_reduce(const float *hpart, int nblocks, int nh, double *H) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nh) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += (double)hpart[(size_t)b * nh + k];
+    H[k] += s;
+}
+
+// dL/dr_i of one frame given G = dL/dH: sum over all j != i of
+// (sum_k G[c][k] K'(d - r_k)) d_ij / d, per (i-tile, j-chunk)
+__global__ void __launch_bounds__(ADJ_BI) k_adj_rdf_grad(AdjK K, const float4 *pos, const float *G, float4 *part) {
+    __shared__ float4 sp[ADJ_JC];
+    __shared__ float sg[ADJ_MAX_CLS * ADJ_MAX_BINS];
+    const int nh = K.ncls * K.bins;
+    for (int t = threadIdx.x; t < nh; t += ADJ_BI) sg[t] = G[t];
+    const int i = blockIdx.x * ADJ_BI + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * ADJ_JC;
+    const int jn = min(ADJ_JC, K.N - j0);
+    for (int t = threadIdx.x; t < jn; t += ADJ_BI) sp[t] = pos[j0 + t];
+    __syncthreads();
+    if (i >= K.N) return;
+    const float4 ri = pos[i];
+    const int ti = adj_type(ri);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int t = 0; t < jn; ++t) {
+        if (j0 + t == i) continue;
+        const float4 rj = sp[t];
+        float dx, dy, dz;
+        const float d2 = dist_sq_mic(ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, K.L0, K.L1, K.L2, dx, dy, dz);
+        if (d2 >= K.rcut2) continue;
+        const float d = sqrtf(d2);
+        const int c = K.pc[ti * K.nt + adj_type(rj)].cls;
+        const int k0 = max(0, (int)floorf((d - 5.0f * K.wid) / K.dr - 0.5f));
+        const int k1 = min(K.bins - 1, (int)ceilf((d + 5.0f * K.wid) / K.dr - 0.5f));
+        float dLdd = 0.f;
+        for (int k = k0; k <= k1; ++k) {
+            const float x = d - ((float)k + 0.5f) * K.dr;
+            const float e = __expf(-0.5f * x * x * K.inv_w2);
+            if (e - K.kfloor > 0.f) dLdd += sg[c * K.bins + k] * (-x * K.inv_w2) * e;
+        }
+        const float sc = dLdd / d;
+        ax += sc * dx;
+        ay += sc * dy;
+        az += sc * dz;
+    }
+    part[(size_t)chunk * K.N + i] = make_float4(ax, ay, az, 0.f);
+}
+
+// backward step n: lambda_r += J w (+ frame gradient); then, unless n == 0,
+// lambda_v += dt lambda_r and w = dt/m lambda_v for step n-1.  Block 0 also
+// adds this step's parameter-gradient block partials (fixed order).
+__global__ void __launch_bounds__(256) k_adj_back(AdjK K, const float4 *vel, float4 *lr, float4 *lv, float4 *w,
+                                                  const float4 *part, const float4 *rpart, int advance,
+                                                  const double *pgrad, int npg, double *pgrad_tot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 2 * ADJ_MAX_CLS && pgrad) {
+        double s = 0.0;
+        for (int b = 0; b < npg; ++b) s += pgrad[(size_t)b * (2 * ADJ_MAX_CLS) + threadIdx.x];
+        pgrad_tot[threadIdx.x] += s;
+    }
+    if (i >= K.N) return;
+    float4 a = lr[i];
+    if (part)
+        for (int c = 0; c < K.nchunk; ++c) {
+            const float4 p = part[(size_t)c * K.N + i];
+            a.x += p.x;
+            a.y += p.y;
+            a.z += p.z;
+        }
+    if (rpart)
+        for (int c = 0; c < K.nchunk; ++c) {
+            const float4 p = rpart[(size_t)c * K.N + i];
+            a.x += p.x;
+            a.y += p.y;
+            a.z += p.z;
+        }
+    lr[i] = a;
+    if (advance) {
+        float4 b = lv[i];
+        b.x += K.dt * a.x;
+        b.y += K.dt * a.y;
+        b.z += K.dt * a.z;
+        lv[i] = b;
+        const float f = K.dt / vel[i].w;
+        w[i] = make_float4(f * b.x, f * b.y, f * b.z, 0.f);
+    }
+}
+
+// ============================================================================
+// Host side
+// ============================================================================
+struct gmd_adj {
+    gmd_adj_desc d;
+    AdjK K;
+    std::vector<int> ptype;
+    std::vector<float> mass;
+    int nck = 0, gi = 0, nblk = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float fwd_ms = 0.f, bwd_ms = 0.f;
+    bool have_forward = false;
+    // device
+    float4 *pos = nullptr, *vel = nullptr, *part = nullptr, *rpart = nullptr;
+    float4 *ck_r = nullptr, *ck_v = nullptr, *seg = nullptr;
+    float4 *lr = nullptr, *lv = nullptr, *w = nullptr;
+    double *pgrad = nullptr, *pgrad_tot = nullptr, *H = nullptr;
+    float *hpart = nullptr, *G = nullptr;
+};
+
+static int adj_free(gmd_adj *a) {
+    if (!a) return 0;
+    void *ptrs[] = {a->pos, a->vel, a->part, a->rpart, a->ck_r, a->ck_v, a->seg, a->lr, a->lv, a->w,
+                    a->pgrad, a->pgrad_tot, a->H, a->hpart, a->G};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (a->e0) hipEventDestroy(a->e0);
+    if (a->e1) hipEventDestroy(a->e1);
+    if (a->stream) hipStreamDestroy(a->stream);
+    delete a;
+    return 0;
+}
+
+static int adj_cls(int a, int b, int nt) {
+    if (a > b) std::swap(a, b);
+    return a * nt - a * (a - 1) / 2 + (b - a);
+}
+
+// pair constants from (sigma, epsilon, cut) per ordered type pair
+static int adj_set_params(gmd_adj *a, const float *params) {
+    const int nt = a->d.ntypes;
+    for (int ti = 0; ti < nt; ++ti)
+        for (int tj = 0; tj < nt; ++tj) {
+            const float *p = params + 3 * (ti * nt + tj);
+            const float *q = params + 3 * (tj * nt + ti);
+            if (p[0] != q[0] || p[1] != q[1] || p[2] != q[2])
+                return fail("gmd_adj: parameters of types (%d,%d) and (%d,%d) differ", ti, tj, tj, ti);
+            if (!(p[0] > 0.f) || !(p[2] > 0.f)) return fail("gmd_adj: sigma and cut must be positive");
+            const float half = 0.5f * std::min(a->d.box[0], std::min(a->d.box[1], a->d.box[2]));
+            if (!(p[2] < half))
+                return fail("gmd_adj: cut %g of types (%d,%d) must be below half the box (%g)", p[2], ti, tj, half);
+            AdjPairC &c = a->K.pc[ti * nt + tj];
+            const double sig = p[0], eps = p[1], rc = p[2];
+            c.sig = (float)sig;
+            c.sig2 = (float)(sig * sig);
+            c.eps = (float)eps;
+            c.eps24 = (float)(24.0 * eps);
+            c.cut2 = (float)(rc * rc);
+            const double x6c = std::pow(sig / rc, 6.0), x12c = x6c * x6c;
+            c.sc = (float)(24.0 * eps * (2.0 * x12c - x6c) / (rc * rc));
+            c.dsc_dsig = (float)(24.0 * eps / sig * (24.0 * x12c - 6.0 * x6c) / (rc * rc));
+            c.cls = adj_cls(ti, tj, nt);
+        }
+    return 0;
+}
+
+extern "C" {
+
+int gmd_adj_create(const gmd_adj_desc *desc, const int *ptype, const float *mass, gmd_adj **out) {
+    if (!desc || !ptype || !mass || !out) return fail("gmd_adj_create: null argument");
+    *out = nullptr;
+    const gmd_adj_desc &d = *desc;
+    if (d.N < 2) return fail("gmd_adj_create: N must be at least 2");
+    if (d.ntypes < 1 || d.ntypes > ADJ_MAX_TYPES) return fail("gmd_adj_create: ntypes must be in [1,%d]", ADJ_MAX_TYPES);
+    if (d.nsteps < 1 || d.ckpt_every < 1) return fail("gmd_adj_create: nsteps and ckpt_every must be positive");
+    if (d.rdf_every < 1 || d.rdf_bins < 1 || !(d.rdf_rmax > 0.f) || !(d.rdf_width > 0.f))
+        return fail("gmd_adj_create: bad RDF settings");
+    const int ncls = d.ntypes * (d.ntypes + 1) / 2;
+    if (ncls * d.rdf_bins > ADJ_MAX_CLS * ADJ_MAX_BINS)
+        return fail("gmd_adj_create: %d classes x %d bins exceed the LDS histogram (%d)", ncls, d.rdf_bins,
+                    ADJ_MAX_CLS * ADJ_MAX_BINS);
+    for (int k = 0; k < 3; ++k)
+        if (!(d.box[k] > 0.f)) return fail("gmd_adj_create: box lengths must be positive");
+    const float half = 0.5f * std::min(d.box[0], std::min(d.box[1], d.box[2]));
+    if (d.rdf_rmax + 5.0f * d.rdf_width > half)
+        return fail("gmd_adj_create: rdf_rmax + 5 width (%g) must stay below half the box (%g)",
+                    d.rdf_rmax + 5.0f * d.rdf_width, half);
+    gmd_adj *a = new gmd_adj();
+    a->d = d;
+    a->ptype.assign(ptype, ptype + d.N);
+    a->mass.assign(mass, mass + d.N);
+    for (int i = 0; i < d.N; ++i) {
+        if (ptype[i] < 0 || ptype[i] >= d.ntypes) {
+            adj_free(a);
+            return fail("gmd_adj_create: particle %d has type %d", i, ptype[i]);
+        }
+        if (!(mass[i] > 0.f)) {
+            adj_free(a);
+            return fail("gmd_adj_create: particle %d has mass %g", i, mass[i]);
+        }
+    }
+    AdjK &K = a->K;
+    memset(&K, 0, sizeof K);
+    K.N = d.N;
+    K.nt = d.ntypes;
+    K.ncls = ncls;
+    K.nchunk = (d.N + ADJ_JC - 1) / ADJ_JC;
+    K.L0 = d.box[0];
+    K.L1 = d.box[1];
+    K.L2 = d.box[2];
+    K.dt = d.dt;
+    K.bins = d.rdf_bins;
+    K.rmax = d.rdf_rmax;
+    K.dr = d.rdf_rmax / d.rdf_bins;
+    K.wid = d.rdf_width;
+    K.inv_w2 = 1.0f / (d.rdf_width * d.rdf_width);
+    K.kfloor = std::exp(-12.5f);
+    const float rc = d.rdf_rmax + 5.0f * d.rdf_width;
+    K.rcut2 = rc * rc;
+    a->gi = (d.N + ADJ_BI - 1) / ADJ_BI;
+    a->nblk = a->gi * K.nchunk;
+    a->nck = (d.nsteps + d.ckpt_every - 1) / d.ckpt_every;
+    const size_t N = d.N;
+#define ATRY(x)                                                               \
+    do {                                                                      \
+        hipError_t e_ = (x);                                                  \
+        if (e_ != hipSuccess) {                                               \
+            adj_free(a);                                                      \
+            return fail("gmd_adj_create: %s: %s", #x, hipGetErrorString(e_)); \
+        }                                                                     \
+    } while (0)
+    ATRY(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+    ATRY(hipEventCreate(&a->e0));
+    ATRY(hipEventCreate(&a->e1));
+    ATRY(hipMalloc((void **)&a->pos, N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->vel, N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->part, (size_t)K.nchunk * N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->rpart, (size_t)K.nchunk * N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->ck_r, (size_t)a->nck * N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->ck_v, (size_t)a->nck * N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->seg, (size_t)(d.ckpt_every + 1) * N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->lr, N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->lv, N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->w, N * sizeof(float4)));
+    ATRY(hipMalloc((void **)&a->pgrad, (size_t)a->nblk * 2 * ADJ_MAX_CLS * sizeof(double)));
+    ATRY(hipMalloc((void **)&a->pgrad_tot, 2 * ADJ_MAX_CLS * sizeof(double)));
+    ATRY(hipMalloc((void **)&a->H, (size_t)ncls * d.rdf_bins * sizeof(double)));
+    ATRY(hipMalloc((void **)&a->hpart, (size_t)a->nblk * ncls * d.rdf_bins * sizeof(float)));
+    ATRY(hipMalloc((void **)&a->G, (size_t)ncls * d.rdf_bins * sizeof(float)));
+#undef ATRY
+    *out = a;
+    return 0;
+}
+
+int gmd_adj_destroy(gmd_adj *a) { return adj_free(a); }
+
+static void adj_rdf_frame(gmd_adj *a, const float4 *pos) {
+    const int nh = a->K.ncls * a->K.bins;
+    hipLaunchKernelGGL(k_adj_rdf, dim3(a->gi, a->K.nchunk), dim3(ADJ_BI), 0, a->stream, a->K, pos, a->hpart);
+    hipLaunchKernelGGL(k_adj_rdf_reduce, dim3((nh + 255) / 256), dim3(256), 0, a->stream, a->hpart, a->nblk, nh,
+                       a->H);
+}
+
+// one forward step from (pos, vel); rec receives r_{n+1} when not null
+static void adj_step(gmd_adj *a, float4 *rec) {
+    hipLaunchKernelGGL(k_adj_pairs<0>, dim3(a->gi, a->K.nchunk), dim3(ADJ_BI), 0, a->stream, a->K, a->pos,
+                       (const float4 *)nullptr, a->part, (double *)nullptr);
+    hipLaunchKernelGGL(k_adj_step, dim3((a->K.N + 255) / 256), dim3(256), 0, a->stream, a->K, a->pos, a->vel,
+                       a->part, rec);
+}
+
+int gmd_adj_forward(gmd_adj *a, const float *params, const float *r0, const float *v0, double *H, float *r_end,
+                    float *v_end) {
+    if (!a || !params || !r0 || !v0 || !H) return fail("gmd_adj_forward: null argument");
+    if (adj_set_params(a, params)) return -1;
+    const int N = a->d.N, S = a->d.nsteps, C = a->d.ckpt_every, E = a->d.rdf_every;
+    std::vector<float4> hr(N), hv(N);
+    for (int i = 0; i < N; ++i) {
+        int t = a->ptype[i];
+        float tf;
+        memcpy(&tf, &t, 4);
+        hr[i] = make_float4(r0[3 * i], r0[3 * i + 1], r0[3 * i + 2], tf);
+        hv[i] = make_float4(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2], a->mass[i]);
+    }
+    hipStream_t st = a->stream;
+    HIPCHECK(hipMemcpyAsync(a->pos, hr.data(), N * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(a->vel, hv.data(), N * sizeof(float4), hipMemcpyHostToDevice, st));
+    const int nh = a->K.ncls * a->K.bins;
+    HIPCHECK(hipMemsetAsync(a->H, 0, nh * sizeof(double), st));
+    HIPCHECK(hipEventRecord(a->e0, st));
+    for (int n = 0; n < S; ++n) {
+        if (n % C == 0) {
+            HIPCHECK(hipMemcpyAsync(a->ck_r + (size_t)(n / C) * N, a->pos, N * sizeof(float4),
+                                    hipMemcpyDeviceToDevice, st));
+            HIPCHECK(hipMemcpyAsync(a->ck_v + (size_t)(n / C) * N, a->vel, N * sizeof(float4),
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        if (n % E == 0) adj_rdf_frame(a, a->pos);
+        adj_step(a, nullptr);
+    }
+    if (S % E == 0) adj_rdf_frame(a, a->pos);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(a->e1, st));
+    HIPCHECK(hipEventSynchronize(a->e1));
+    HIPCHECK(hipEventElapsedTime(&a->fwd_ms, a->e0, a->e1));
+    HIPCHECK(hipMemcpy(H, a->H, nh * sizeof(double), hipMemcpyDeviceToHost));
+    if (r_end || v_end) {
+        HIPCHECK(hipMemcpy(hr.data(), a->pos, N * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hv.data(), a->vel, N * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            if (r_end) {
+                r_end[3 * i] = hr[i].x;
+                r_end[3 * i + 1] = hr[i].y;
+                r_end[3 * i + 2] = hr[i].z;
+            }
+            if (v_end) {
+                v_end[3 * i] = hv[i].x;
+                v_end[3 * i + 1] = hv[i].y;
+                v_end[3 * i + 2] = hv[i].z;
+            }
+        }
+    }
+    a->have_forward = true;
+    return 0;
+}
+
+int gmd_adj_backward(gmd_adj *a, const double *dLdH, double *grad_params, float *grad_r0, float *grad_v0) {
+    if (!a || !dLdH || !grad_params || !grad_r0) return fail("gmd_adj_backward: null argument");
+    if (!a->have_forward) return fail("gmd_adj_backward: no forward pass to differentiate");
+    const int N = a->d.N, S = a->d.nsteps, C = a->d.ckpt_every, E = a->d.rdf_every;
+    const int nh = a->K.ncls * a->K.bins;
+    hipStream_t st = a->stream;
+    std::vector<float> g(nh);
+    for (int k = 0; k < nh; ++k) g[k] = (float)dLdH[k];
+    HIPCHECK(hipMemcpyAsync(a->G, g.data(), nh * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemsetAsync(a->lr, 0, N * sizeof(float4), st));
+    HIPCHECK(hipMemsetAsync(a->lv, 0, N * sizeof(float4), st));
+    HIPCHECK(hipMemsetAsync(a->pgrad_tot, 0, 2 * ADJ_MAX_CLS * sizeof(double), st));
+    HIPCHECK(hipEventRecord(a->e0, st));
+    const dim3 gp(a->gi, a->K.nchunk), gu((N + 255) / 256);
+    // lambda_r(S) = dL/dr_S, then lambda_v, w of step S-1; recompute r_S from the last checkpoint first
+    const int last = a->nck - 1;
+    // recompute the last segment (it also provides r_S)
+    bool have_seg = false;
+    int seg_n0 = -1;
+    for (int sgm = last; sgm >= 0; --sgm) {
+        const int n0 = sgm * C, n1 = std::min(S, n0 + C);
+        HIPCHECK(hipMemcpyAsync(a->pos, a->ck_r + (size_t)sgm * N, N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(a->vel, a->ck_v + (size_t)sgm * N, N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(a->seg, a->pos, N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        for (int k = 0; k < n1 - n0; ++k) adj_step(a, a->seg + (size_t)(k + 1) * N);
+        if (!have_seg) {
+            // start of the adjoint: frame gradient at r_S, then lambda_v and w of step S-1
+            const float4 *rS = a->seg + (size_t)(n1 - n0) * N;
+            const float4 *rp = nullptr;
+            if (S % E == 0) {
+                hipLaunchKernelGGL(k_adj_rdf_grad, gp, dim3(ADJ_BI), 0, st, a->K, rS, a->G, a->rpart);
+                rp = a->rpart;
+            }
+            hipLaunchKernelGGL(k_adj_back, gu, dim3(256), 0, st, a->K, a->vel, a->lr, a->lv, a->w,
+                               (const float4 *)nullptr, rp, 1, (const double *)nullptr, 0, a->pgrad_tot);
+            have_seg = true;
+        }
+        seg_n0 = n0;
+        for (int n = n1 - 1; n >= n0; --n) {
+            const float4 *rn = a->seg + (size_t)(n - seg_n0) * N;
+            hipLaunchKernelGGL(k_adj_pairs<1>, gp, dim3(ADJ_BI), 0, st, a->K, rn, (const float4 *)a->w, a->part,
+                               a->pgrad);
+            const float4 *rp = nullptr;
+            if (n % E == 0) {
+                hipLaunchKernelGGL(k_adj_rdf_grad, gp, dim3(ADJ_BI), 0, st, a->K, rn, a->G, a->rpart);
+                rp = a->rpart;
+            }
+            hipLaunchKernelGGL(k_adj_back, gu, dim3(256), 0, st, a->K, a->vel, a->lr, a->lv, a->w,
+                               (const float4 *)a->part, rp, n > 0 ? 1 : 0, (const double *)a->pgrad, a->nblk,
+                               a->pgrad_tot);
+        }
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(a->e1, st));
+    HIPCHECK(hipEventSynchronize(a->e1));
+    HIPCHECK(hipEventElapsedTime(&a->bwd_ms, a->e0, a->e1));
+    // gradient per unordered class: (d/dsigma, d/depsilon)
+    double pg[2 * ADJ_MAX_CLS];
+    HIPCHECK(hipMemcpy(pg, a->pgrad_tot, sizeof pg, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 2 * a->K.ncls; ++c) grad_params[c] = pg[c];
+    std::vector<float4> h(N);
+    HIPCHECK(hipMemcpy(h.data(), a->lr, N * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < N; ++i) {
+        grad_r0[3 * i] = h[i].x;
+        grad_r0[3 * i + 1] = h[i].y;
+        grad_r0[3 * i + 2] = h[i].z;
+    }
+    if (grad_v0) {
+        HIPCHECK(hipMemcpy(h.data(), a->lv, N * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            grad_v0[3 * i] = h[i].x;
+            grad_v0[3 * i + 1] = h[i].y;
+            grad_v0[3 * i + 2] = h[i].z;
+        }
+    }
+    return 0;
+}
+
+int gmd_adj_last_ms(gmd_adj *a, float *fwd_ms, float *bwd_ms) {
+    if (!a) return fail("gmd_adj_last_ms: null handle");
+    if (fwd_ms) *fwd_ms = a->fwd_ms;
+    if (bwd_ms) *bwd_ms = a->bwd_ms;
+    return 0;
+}
+
+}  // extern "C"

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -15,9 +16,10 @@
 // ============================================================================
 // Drivers
 // ============================================================================
+// Every step of a run inside one launch; the blocks of one system meet at a
+// software barrier after each step, and the last to arrive runs the tail.
 template <int BASE, int CUT, int INTEG>
-__global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
-    const DP &P = *Pp;
+__device__ inline void persistent_body(const DP &P, const BlockCtx &B) {
     extern __shared__ __align__(16) char lds[];
     PairP *pp = (PairP *)(lds + P.lds_pp);
     double *red = (double *)(lds + P.lds_red);
@@ -31,19 +33,17 @@ __global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
         float acc[GMD_NRED];
 #pragma unroll
         for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
-        step_body<BASE, CUT, INTEG>(P, I, pp, build, acc);
-        if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, acc, red);
-        grid_barrier(P.ctrl, [&]() { step_tail(P, I, red); });
+        step_body<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
+        if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, B, acc, red);
+        grid_barrier(P.ctrl, B.nb, [&]() { step_tail(P, I, red, B.nb); });
     }
 }
 
-// One step per launch; the step index and the rebuild decision come from the
-// control block written by the previous launch's last block.
+// One step; the step index and the rebuild decision come from the control
+// block that the previous step's tail wrote.
 template <int BASE, int CUT, int INTEG>
-__global__ void __launch_bounds__(256) k_step(const DP *__restrict__ Pp) {
-    const DP &P = *Pp;
+__device__ inline void step_once(const DP &P, const BlockCtx &B) {
     extern __shared__ __align__(16) char lds[];
-    __shared__ int s_last;
     const int s = P.ctrl->step;
     if (s > P.ctrl->nsteps) return;
     const bool rebuild = P.ctrl->rebuild_now != 0;
@@ -56,19 +56,65 @@ __global__ void __launch_bounds__(256) k_step(const DP *__restrict__ Pp) {
     float acc[GMD_NRED];
 #pragma unroll
     for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
-    step_body<BASE, CUT, INTEG>(P, I, pp, build, acc);
-    if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, acc, red);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned a =
-            __hip_atomic_fetch_add(gen_ptr(&P.ctrl->red_count), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (a == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) {
-        step_tail(P, I, red);
-        if (threadIdx.x == 0) P.ctrl->red_count = 0;
-    }
+    step_body<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
+    if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, B, acc, red);
+}
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
+    persistent_body<BASE, CUT, INTEG>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
+}
+
+// One step per launch.  The cross-block work (sums, thermostat, momentum
+// shift, saver row, rebuild decision) is the separate one-block k_tail: a
+// last-block-done scheme here would need an agent-scope release per block,
+// each a write-back of the XCD's L2 under the blocks still running, which
+// costs more than one extra launch.
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_step(const DP *__restrict__ Pp) {
+    step_once<BASE, CUT, INTEG>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
+}
+
+// Batched replicas: block b of the grid works for replica blk[b].rep as its
+// block blk[b].bid of blk[b].nb.  Replicas never wait for each other: the
+// persistent form synchronises only the blocks of one replica.
+struct BatchBlk {
+    int rep, bid, nb, pad;
+};
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_batch_persistent(const DP *const *Ps, const BatchBlk *blks) {
+    const BatchBlk a = blks[blockIdx.x];
+    if (a.rep < 0) return;
+    persistent_body<BASE, CUT, INTEG>(*Ps[a.rep], BlockCtx{a.bid, a.nb});
+}
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_batch_step(const DP *const *Ps, const BatchBlk *blks) {
+    const BatchBlk a = blks[blockIdx.x];
+    if (a.rep < 0) return;
+    step_once<BASE, CUT, INTEG>(*Ps[a.rep], BlockCtx{a.bid, a.nb});
+}
+
+// The cross-block tail of a step as its own one-block kernel (the kernel
+// boundary orders it after every block of k_step).
+__global__ void __launch_bounds__(256) k_tail(const DP *__restrict__ Pp, int integrator) {
+    const DP &P = *Pp;
+    __shared__ double red[4 * GMD_NRED];
+    const int s = P.ctrl->step;
+    if (s > P.ctrl->nsteps) return;
+    const StepInfo I = step_info(P, s, integrator, P.ctrl->rebuild_now != 0);
+    step_tail(P, I, red, P.step_grid);
+}
+
+// the tails of all replicas of a batch: block r finishes replica r's step
+__global__ void __launch_bounds__(256) k_batch_tail(const DP *const *Ps, int integrator) {
+    const DP &P = *Ps[blockIdx.x];
+    __shared__ double red[4 * GMD_NRED];
+    const int s = P.ctrl->step;
+    if (s > P.ctrl->nsteps) return;
+    const StepInfo I = step_info(P, s, integrator, P.ctrl->rebuild_now != 0);
+    step_tail(P, I, red, P.step_grid);
 }
 
 // ---- run set-up: cell grid of the starting positions and the skin test ----
@@ -247,38 +293,40 @@ int dalloc(T **p, size_t n) {
 }
 
 typedef void (*kfun)(const DP *);
+typedef void (*bfun)(const DP *const *, const BatchBlk *);
+
+// the kernel instances of one (pair function, cutoff, integrator) combination
+struct Kernels {
+    kfun pers, step;
+    bfun bpers, bstep;
+};
 
 template <int BASE, int CUT, int INTEG>
-void pick(kfun &pers, kfun &step) {
-    pers = k_persistent<BASE, CUT, INTEG>;
-    step = k_step<BASE, CUT, INTEG>;
+Kernels pick() {
+    return Kernels{k_persistent<BASE, CUT, INTEG>, k_step<BASE, CUT, INTEG>, k_batch_persistent<BASE, CUT, INTEG>,
+                   k_batch_step<BASE, CUT, INTEG>};
 }
 
 template <int BASE, int CUT>
-void pick_integ(int integ, kfun &pers, kfun &step) {
+Kernels pick_integ(int integ) {
     switch (integ) {
-        case 0: pick<BASE, CUT, 0>(pers, step); break;
-        case 1: pick<BASE, CUT, 1>(pers, step); break;
-        case 2: pick<BASE, CUT, 2>(pers, step); break;
-        default: pick<BASE, CUT, 3>(pers, step); break;
+        case 0: return pick<BASE, CUT, 0>();
+        case 1: return pick<BASE, CUT, 1>();
+        case 2: return pick<BASE, CUT, 2>();
+        default: return pick<BASE, CUT, 3>();
     }
 }
 
 template <int BASE>
-void pick_cut(int cut, int integ, kfun &pers, kfun &step) {
+Kernels pick_cut(int cut, int integ) {
     switch (cut) {
-        case 0: pick_integ<BASE, 0>(integ, pers, step); break;
-        case 1: pick_integ<BASE, 1>(integ, pers, step); break;
-        default: pick_integ<BASE, 2>(integ, pers, step); break;
+        case 0: return pick_integ<BASE, 0>(integ);
+        case 1: return pick_integ<BASE, 1>(integ);
+        default: return pick_integ<BASE, 2>(integ);
     }
 }
 
-void pick_kernels(int base, int cut, int integ, kfun &pers, kfun &step) {
-    if (base == 0)
-        pick_cut<0>(cut, integ, pers, step);
-    else
-        pick_cut<1>(cut, integ, pers, step);
-}
+Kernels pick_kernels(int base, int cut, int integ) { return base == 0 ? pick_cut<0>(cut, integ) : pick_cut<1>(cut, integ); }
 
 float i2f(int v) {
     float f;
@@ -311,6 +359,7 @@ struct gmd_ctx {
     std::vector<float4> h_pos, h_vel;
     std::vector<int> h_rim;
     hipStream_t stream = nullptr;
+    bool own_stream = true;   // false: a replica of a gmd_batch (the batch owns the stream and the context)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0;
     // device
@@ -334,7 +383,7 @@ struct gmd_ctx {
     // launch
     DP P;
     DP *d_P = nullptr;            // device copy of P (kernels take a pointer)
-    kfun k_pers = nullptr, k_stp = nullptr;
+    Kernels kern = {nullptr, nullptr, nullptr, nullptr};
     int block = 256, grid_pers = 1, grid_step = 1, grid_aux = 1;
     size_t lds_pers = 0, lds_step = 0, lds_build = 0;
     hipGraphExec_t gexec = nullptr;
@@ -388,12 +437,18 @@ static int ctx_free(gmd_ctx *c) {
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
 
-int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
+static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_stream);
+
+int gmd_create(const gmd_desc *desc, gmd_ctx **out) { return ctx_create(desc, out, nullptr); }
+
+// shared_stream != nullptr: the context works on the caller's stream (a batch
+// of replicas) instead of creating its own
+static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_stream) {
     if (!desc || !out) return fail("gmd_create: null argument");
     *out = nullptr;
     const gmd_desc &d = *desc;
@@ -446,7 +501,12 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     hipDeviceProp_t prop;
     HTRY(hipGetDeviceProperties(&prop, dev));
     c->cus = prop.multiProcessorCount;
-    HTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (shared_stream) {
+        c->stream = shared_stream;
+        c->own_stream = false;
+    } else {
+        HTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
     HTRY(hipEventCreate(&c->ev0));
     HTRY(hipEventCreate(&c->ev1));
 
@@ -464,6 +524,10 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
         }
     }
     const int ncells = ncx * ncy * ncz;
+    if (d.nblist_kind == GMD_NBLIST_LINKED_LISTS && N >= LL_MAX_N) {
+        ctx_free(c);
+        return fail("linked lists support at most %d particles (got %d)", LL_MAX_N - 1, N);
+    }
 
     // ---- launch geometry ----
     const int tp = d.tp;
@@ -475,9 +539,13 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     c->grid_step = (N + slots - 1) / slots;
     c->grid_aux = std::max(1, std::min((N + block - 1) / block, c->cus * 16));
     const int wpb = block / 64;
-    // dynamic LDS: pair table (4 KB) | reduction scratch (2 KB) | rebuild scratch
-    c->lds_build = std::max((size_t)wpb * LL_WAVE_LDS, (size_t)block * 16);
-    const size_t lds_pp = 0, lds_red = 4096, lds_b = 4096 + 16 * 16 * 8;
+    // dynamic LDS: pair table | reduction scratch (one row of sums per wave) |
+    // rebuild scratch; sized so that four blocks of 256 fit in a CU's 160 KB
+    const size_t lds_cells = (d.nblist_kind == GMD_NBLIST_LINKED_LISTS) ? (size_t)wpb * LL_WAVE_LDS : 0;
+    c->lds_build = std::max(lds_cells, (size_t)block * 16);
+    const size_t lds_pp = 0;
+    const size_t lds_red = ((size_t)d.ntypes * d.ntypes * sizeof(PairP) + 15) / 16 * 16;
+    const size_t lds_b = lds_red + (size_t)wpb * GMD_NRED * sizeof(double);
     c->lds_step = lds_b + c->lds_build;
     c->lds_pers = c->lds_step;
     // cell capacity: room for 4x the mean occupancy, at least 16, at most 64
@@ -492,7 +560,8 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     }
 
     // ---- buffers ----
-    const size_t rows = (size_t)(d.max_nbs + tp - 1) / tp;
+    // at least 8 rows: the force loop requests the first 8 rows before the count arrives
+    const size_t rows = std::max((size_t)(d.max_nbs + tp - 1) / tp, (size_t)8);
     c->nbl_len = rows * (size_t)N * tp;
     TRY(dalloc(&c->pos0, N));
     TRY(dalloc(&c->pos1, N));
@@ -520,6 +589,8 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     HTRY(hipMemset(c->sc, 0, (size_t)5 * N * sizeof(float)));
     HTRY(hipMemset(c->rim, 0, (size_t)3 * N * sizeof(int)));
     HTRY(hipMemset(c->nbc, 0, (size_t)N * sizeof(int)));
+    HTRY(hipMemset(c->nbl, 0, c->nbl_len * sizeof(int)));   // entries past a count stay valid indices
+    HTRY(hipMemset(c->cmem, 0, (size_t)3 * ncells * ccap * sizeof(int)));
     HTRY(hipMemset(c->obeta, 0, (size_t)N * sizeof(float4)));
     HTRY(hipMemset(c->frc, 0, (size_t)N * sizeof(float4)));
     HTRY(hipMemset(c->rref, 0, (size_t)N * sizeof(float4)));
@@ -596,14 +667,15 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
     P.lds_pp = (int)lds_pp;
     P.lds_red = (int)lds_red;
     P.lds_build = (int)lds_b;
+    P.step_grid = c->grid_step;
 
     TRY(dalloc(&c->d_P, 1));
-    pick_kernels(d.pair_base, d.cutoff, d.integrator, c->k_pers, c->k_stp);
+    c->kern = pick_kernels(d.pair_base, d.cutoff, d.integrator);
 
     // ---- cooperative grid: all blocks must be co-resident ----
     if (d.gridsync) {
         int nb = 0;
-        HTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)c->k_pers, block, c->lds_pers));
+        HTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)c->kern.pers, block, c->lds_pers));
         if (nb < 1) {
             ctx_free(c);
             return fail("gridsync: persistent kernel cannot be resident with %d threads per block", block);
@@ -616,7 +688,10 @@ int gmd_create(const gmd_desc *desc, gmd_ctx **out) {
 #undef HTRY
 }
 
-int gmd_destroy(gmd_ctx *ctx) { return ctx_free(ctx); }
+int gmd_destroy(gmd_ctx *ctx) {
+    if (ctx && !ctx->own_stream) return fail("gmd_destroy: this context belongs to a batch; use gmd_batch_destroy");
+    return ctx_free(ctx);
+}
 
 int gmd_upload(gmd_ctx *c, const float *r, const float *v, const float *f, const float *scalars, const int *ptype,
                const int *r_im) {
@@ -746,7 +821,9 @@ static int ensure_flags(gmd_ctx *c, int steps) {
     return 0;
 }
 
-static int run_impl(gmd_ctx *c, float t0, int steps) {
+// per-run set-up of one system on its stream: control block, rebuild flags,
+// saver rows, the cell grid of the starting positions and the skin test
+static int run_prepare(gmd_ctx *c, float t0, int steps) {
     if (steps < 0) return fail("gmd_run: steps must be non-negative");
     if (ensure_flags(c, steps)) return -1;
     hipStream_t st = c->stream;
@@ -765,30 +842,11 @@ static int run_impl(gmd_ctx *c, float t0, int steps) {
     hipLaunchKernelGGL(k_run_setup, dim3(c->grid_aux), dim3(c->block), 0, st, dP);
     hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(64), 0, st, dP);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(c->ev0, st));
-    if (c->d.gridsync) {
-        void *args[] = {(void *)&dP};
-        HIPCHECK(hipLaunchCooperativeKernel((const void *)c->k_pers, dim3(c->grid_pers), dim3(c->block), args,
-                                            (unsigned)c->lds_pers, st));
-    } else if (c->d.use_graph) {
-        if (!c->gexec) {
-            HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < GRAPH_STEPS; ++k)
-                hipLaunchKernelGGL(c->k_stp, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
-            HIPCHECK(hipStreamEndCapture(st, &c->graph));
-            HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-        }
-        const int launches = (steps + 1 + GRAPH_STEPS - 1) / GRAPH_STEPS;
-        for (int k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(c->gexec, st));
-    } else {
-        for (int s = 0; s <= steps; ++s)
-            hipLaunchKernelGGL(c->k_stp, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
-        HIPCHECK(hipGetLastError());
-    }
-    HIPCHECK(hipEventRecord(c->ev1, st));
-    HIPCHECK(hipEventSynchronize(c->ev1));
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return 0;
+}
+
+// after the stream has drained: host-side state of the run and its errors
+static int run_complete(gmd_ctx *c, int steps) {
     c->buf0 ^= (steps & 1);
     c->lstep += steps;
     c->need_rebuild = false;
@@ -800,8 +858,44 @@ static int run_impl(gmd_ctx *c, float t0, int steps) {
     return 0;
 }
 
+static int run_impl(gmd_ctx *c, float t0, int steps) {
+    if (run_prepare(c, t0, steps)) return -1;
+    hipStream_t st = c->stream;
+    const DP *dP = c->d_P;
+    HIPCHECK(hipEventRecord(c->ev0, st));
+    if (c->d.gridsync) {
+        void *args[] = {(void *)&dP};
+        HIPCHECK(hipLaunchCooperativeKernel((const void *)c->kern.pers, dim3(c->grid_pers), dim3(c->block), args,
+                                            (unsigned)c->lds_pers, st));
+    } else if (c->d.use_graph) {
+        if (!c->gexec) {
+            HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < GRAPH_STEPS; ++k) {
+                hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
+                hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, dP, c->d.integrator);
+            }
+            HIPCHECK(hipStreamEndCapture(st, &c->graph));
+            HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        }
+        const int launches = (steps + 1 + GRAPH_STEPS - 1) / GRAPH_STEPS;
+        for (int k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(c->gexec, st));
+    } else {
+        for (int s = 0; s <= steps; ++s) {
+            hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
+            hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, dP, c->d.integrator);
+        }
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipEventRecord(c->ev1, st));
+    HIPCHECK(hipEventSynchronize(c->ev1));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return run_complete(c, steps);
+}
+
 int gmd_run(gmd_ctx *c, float time_zero, int steps) {
     if (!c) return fail("gmd_run: null context");
+    if (!c->own_stream) return fail("gmd_run: this context is a replica of a batch; use gmd_batch_run");
     if (c->d.integrator == GMD_INTEGRATOR_NONE && steps > 0)
         return fail("gmd_run: context has no integrator; only steps=0 (evaluation) is possible");
     return run_impl(c, time_zero, steps);
@@ -809,6 +903,7 @@ int gmd_run(gmd_ctx *c, float time_zero, int steps) {
 
 int gmd_evaluate(gmd_ctx *c) {
     if (!c) return fail("gmd_evaluate: null context");
+    if (!c->own_stream) return fail("gmd_evaluate: this context is a replica of a batch");
     return run_impl(c, 0.0f, 0);
 }
 
@@ -1038,4 +1133,205 @@ int gmd_rdf_compute_host(gmd_rdf *r, const float *pos, const int *ptype, const f
     return rdf_run(r, r->pos, nullptr, box, counts);
 }
 
+
+// ============================================================================
+// Batches of independent replicas
+// ============================================================================
+struct gmd_batch {
+    std::vector<gmd_ctx *> reps;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int driver = GMD_BATCH_PERSISTENT, integrator = 0, block = 256, grid = 0, cus = 0;
+    size_t lds = 0;
+    Kernels kern = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<BatchBlk> h_blk;
+    BatchBlk *d_blk = nullptr;
+    const DP **d_Ps = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    float last_ms = 0.f;
+};
+
+static int batch_free(gmd_batch *b) {
+    if (!b) return 0;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (gmd_ctx *c : b->reps) ctx_free(c);
+    if (b->d_blk) hipFree(b->d_blk);
+    if (b->d_Ps) hipFree(b->d_Ps);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+// Grid layout of a batch.  Consecutive workgroups of a launch go to the
+// device's 8 XCDs in turn, so grid slot x*8 + k runs on XCD k.  Replicas are
+// packed into the XCDs greedily by block count (largest first) and each
+// replica's blocks are placed on slots of one XCD: a replica's positions and
+// lists then live in one XCD's L2.  Slots left over in a short XCD are idle
+// blocks (rep = -1) that return at once.
+static void batch_layout(const std::vector<int> &nb, std::vector<BatchBlk> &out) {
+    const int NX = 8;
+    std::vector<int> order(nb.size());
+    for (size_t r = 0; r < nb.size(); ++r) order[r] = (int)r;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return nb[a] > nb[b2]; });
+    std::vector<std::vector<BatchBlk>> bins(NX);
+    std::vector<long long> load(NX, 0);
+    for (int r : order) {
+        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        for (int k = 0; k < nb[r]; ++k) bins[x].push_back(BatchBlk{r, k, nb[r], 0});
+        load[x] += nb[r];
+    }
+    size_t len = 0;
+    for (auto &v : bins) len = std::max(len, v.size());
+    out.assign(len * NX, BatchBlk{-1, 0, 0, 0});
+    for (int x = 0; x < NX; ++x)
+        for (size_t k = 0; k < bins[x].size(); ++k) out[k * NX + x] = bins[x][k];
+}
+
+int gmd_batch_create(const gmd_desc *descs, int nrep, const int *blocks, int driver, gmd_batch **out) {
+    if (!descs || !out) return fail("gmd_batch_create: null argument");
+    *out = nullptr;
+    if (nrep < 1) return fail("gmd_batch_create: need at least one replica");
+    if (driver != GMD_BATCH_PERSISTENT && driver != GMD_BATCH_GRAPH)
+        return fail("gmd_batch_create: unknown driver %d", driver);
+    const gmd_desc &d0 = descs[0];
+    for (int r = 1; r < nrep; ++r) {
+        const gmd_desc &d = descs[r];
+        if (d.pair_base != d0.pair_base || d.cutoff != d0.cutoff || d.integrator != d0.integrator ||
+            d.ntypes != d0.ntypes || d.tp != d0.tp || d.pb != d0.pb)
+            return fail("gmd_batch_create: replica %d differs from replica 0 in pair function, cutoff, integrator, "
+                        "number of types, tp or pb (one kernel runs every replica)", r);
+    }
+    if (d0.integrator == GMD_INTEGRATOR_NONE) return fail("gmd_batch_create: replicas need an integrator");
+    gmd_batch *b = new gmd_batch();
+    b->driver = driver;
+    b->integrator = d0.integrator;
+#define BTRY(x)                                                             \
+    do {                                                                    \
+        hipError_t e_ = (x);                                                \
+        if (e_ != hipSuccess) {                                             \
+            batch_free(b);                                                  \
+            return fail("gmd_batch_create: %s: %s", #x, hipGetErrorString(e_)); \
+        }                                                                   \
+    } while (0)
+    BTRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BTRY(hipEventCreate(&b->ev0));
+    BTRY(hipEventCreate(&b->ev1));
+    std::vector<int> nb(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        gmd_desc d = descs[r];
+        d.gridsync = 0;
+        d.use_graph = 0;
+        gmd_ctx *c = nullptr;
+        if (ctx_create(&d, &c, b->stream)) {
+            std::string msg = g_err;
+            batch_free(b);
+            return fail("gmd_batch_create: replica %d: %s", r, msg.c_str());
+        }
+        b->reps.push_back(c);
+        nb[r] = blocks ? blocks[r] : c->grid_step;
+        const int cap = std::max(c->grid_step, c->cus * 32);
+        if (nb[r] < 1 || nb[r] > cap) {
+            batch_free(b);
+            return fail("gmd_batch_create: replica %d: %d blocks requested, allowed 1..%d", r, nb[r], cap);
+        }
+        c->P.step_grid = nb[r];
+        BTRY(hipMemcpy(c->d_P, &c->P, sizeof(DP), hipMemcpyHostToDevice));
+        b->lds = std::max(b->lds, c->lds_step);
+        b->block = c->block;
+        b->cus = c->cus;
+    }
+    b->kern = b->reps[0]->kern;
+    batch_layout(nb, b->h_blk);
+    b->grid = (int)b->h_blk.size();
+    if (driver == GMD_BATCH_PERSISTENT) {
+        int per_cu = 0;
+        BTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b->kern.bpers, b->block, b->lds));
+        if (per_cu * b->cus < b->grid) {
+            const int g = b->grid, room = per_cu * b->cus, bs = b->block;
+            batch_free(b);
+            return fail("gmd_batch_create: the persistent batch needs %d co-resident blocks of %d threads, the device "
+                        "holds %d; assign fewer blocks per replica", g, bs, room);
+        }
+    }
+    BTRY(hipMalloc((void **)&b->d_blk, b->h_blk.size() * sizeof(BatchBlk)));
+    BTRY(hipMemcpy(b->d_blk, b->h_blk.data(), b->h_blk.size() * sizeof(BatchBlk), hipMemcpyHostToDevice));
+    std::vector<const DP *> ps(nrep);
+    for (int r = 0; r < nrep; ++r) ps[r] = b->reps[r]->d_P;
+    BTRY(hipMalloc((void **)&b->d_Ps, nrep * sizeof(DP *)));
+    BTRY(hipMemcpy(b->d_Ps, ps.data(), nrep * sizeof(DP *), hipMemcpyHostToDevice));
+#undef BTRY
+    *out = b;
+    return 0;
+}
+
+int gmd_batch_destroy(gmd_batch *b) { return batch_free(b); }
+
+int gmd_batch_size(gmd_batch *b, int *nrep, int *grid) {
+    if (!b) return fail("gmd_batch_size: null batch");
+    if (nrep) *nrep = (int)b->reps.size();
+    if (grid) *grid = b->grid;
+    return 0;
+}
+
+int gmd_batch_context(gmd_batch *b, int rep, gmd_ctx **out) {
+    if (!b || !out) return fail("gmd_batch_context: null argument");
+    if (rep < 0 || rep >= (int)b->reps.size()) return fail("gmd_batch_context: replica %d out of range", rep);
+    *out = b->reps[rep];
+    return 0;
+}
+
+int gmd_batch_run(gmd_batch *b, float time_zero, int steps) {
+    if (!b) return fail("gmd_batch_run: null batch");
+    for (size_t r = 0; r < b->reps.size(); ++r)
+        if (run_prepare(b->reps[r], time_zero, steps)) {
+            std::string msg = g_err;
+            return fail("gmd_batch_run: replica %d: %s", (int)r, msg.c_str());
+        }
+    // the rebuild-flag arrays may have been reallocated by the preparation
+    hipStream_t st = b->stream;
+    HIPCHECK(hipEventRecord(b->ev0, st));
+    const DP *const *Ps = b->d_Ps;
+    const BatchBlk *blk = b->d_blk;
+    if (b->driver == GMD_BATCH_PERSISTENT) {
+        void *args[] = {(void *)&Ps, (void *)&blk};
+        HIPCHECK(hipLaunchCooperativeKernel((const void *)b->kern.bpers, dim3(b->grid), dim3(b->block), args,
+                                            (unsigned)b->lds, st));
+    } else {
+        if (!b->gexec) {
+            HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < GRAPH_STEPS; ++k) {
+                hipLaunchKernelGGL(b->kern.bstep, dim3(b->grid), dim3(b->block), b->lds, st, Ps, blk);
+                hipLaunchKernelGGL(k_batch_tail, dim3((unsigned)b->reps.size()), dim3(256), 0, st, Ps, b->integrator);
+            }
+            HIPCHECK(hipStreamEndCapture(st, &b->graph));
+            HIPCHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        const int launches = (steps + 1 + GRAPH_STEPS - 1) / GRAPH_STEPS;
+        for (int k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(b->gexec, st));
+    }
+    HIPCHECK(hipEventRecord(b->ev1, st));
+    HIPCHECK(hipEventSynchronize(b->ev1));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+    for (size_t r = 0; r < b->reps.size(); ++r)
+        if (run_complete(b->reps[r], steps)) {
+            std::string msg = g_err;
+            return fail("gmd_batch_run: replica %d: %s", (int)r, msg.c_str());
+        }
+    return 0;
+}
+
+int gmd_batch_last_run_ms(gmd_batch *b, float *ms) {
+    if (!b || !ms) return fail("gmd_batch_last_run_ms: null argument");
+    *ms = b->last_ms;
+    return 0;
+}
+
 }  // extern "C"
+
+#include "gmd_adjoint.h"

@@ -96,6 +96,7 @@ struct DP {
     GMEM int *rebuild_flag;    // [nsteps+2]
     GMEM float *saver_out;     // [saves_per_block][6]
     int lds_pp, lds_red, lds_build;   // byte offsets into dynamic LDS
+    int step_grid;             // blocks of the step kernel (partial sums to reduce)
 };
 
 // ---------------------------------------------------------------------------
@@ -103,10 +104,12 @@ struct DP {
 // float32 operation order matters for neighbour sets and histogram bins, so
 // contraction into FMA is switched off here.
 // ---------------------------------------------------------------------------
+// Reference rule: d -= L if 2d > L, d += L if 2d < -L.  Doubling is exact,
+// so this is |d| > L/2 -> d - copysign(L, d): the same result bit for bit,
+// in four instructions (L/2 is loop-invariant).
 __device__ __forceinline__ float mic1(float d, float L) {
 #pragma clang fp contract(off)
-    float t = 2.0f * d;
-    return t > L ? d - L : (t < -L ? d + L : d);
+    return (fabsf(d) > 0.5f * L) ? d - copysignf(L, d) : d;
 }
 
 __device__ __forceinline__ float dist_sq_mic(float ax, float ay, float az, float bx, float by, float bz,
@@ -176,21 +179,22 @@ __device__ inline void make_pairp(int base, int cutoff, float p0, float p1, floa
 
 // Fast evaluation inside the force loop: from d2 directly, one reciprocal.
 template <int BASE, int CUT>
-__device__ __forceinline__ void pair_fast(float d2, const PairP &q, float &u, float &s, float &umm) {
+__device__ __forceinline__ void pair_fast(float d2, const PairP &q, float &u, float &s, float &umm, bool lap) {
     float ir2 = __builtin_amdgcn_rcpf(d2);
+    umm = 0.f;
     if (BASE == 0) {
         float x2 = q.a * ir2;
         float x6 = x2 * x2 * x2;
         float x12 = x6 * x6;
         u = q.b * (x12 - x6);
         s = q.c * (2.0f * x12 - x6) * ir2;
-        umm = q.c * (26.0f * x12 - 7.0f * x6) * ir2;
+        if (lap) umm = q.c * (26.0f * x12 - 7.0f * x6) * ir2;
     } else {
         float p6 = ir2 * ir2 * ir2;
         float p12 = p6 * p6;
         u = q.a * p12 + q.b * p6;
         s = (q.c * p12 + q.d * p6) * ir2;
-        umm = (13.0f * q.c * p12 + 7.0f * q.d * p6) * ir2;
+        if (lap) umm = (13.0f * q.c * p12 + 7.0f * q.d * p6) * ir2;
     }
     if (CUT == 1) {
         u = u - q.ucut;

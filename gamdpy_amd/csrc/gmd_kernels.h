@@ -16,19 +16,27 @@
 // particle can be integrated as soon as its own force is known.  After the
 // update each particle inserts itself into the cell grid of the next step
 // (three rotating buffers), so a rebuild never needs an extra pass or a grid
-// barrier.  The last block of each step finishes the deterministic
+// barrier.  The step's tail (the one-block k_tail kernel, or one block of
+// the persistent kernel after its grid barrier) finishes the deterministic
 // reductions (thermostat, momentum reset, scalar saver) and publishes the
 // rebuild decision of the next step.
 #pragma once
 #include "gmd_device.h"
 
-#define LL_IDXCAP 512   // stencil member indices staged per wave
-#define LL_POSCAP 256   // candidate positions staged per wave
-#define LL_OWNCAP 64    // particles of the centre cell (cell capacity limit)
-#define LL_WAVE_LDS (LL_OWNCAP * 4 + LL_IDXCAP * 4 + LL_POSCAP * 16)
+#define LL_PER 9                 // candidates per lane and pass of the cell walk
+#define LL_IDXCAP (64 * LL_PER)  // stencil members per wave and pass
+#define LL_OWNCAP 64             // particles of the centre cell (cell capacity limit)
+#define LL_WAVE_LDS (LL_OWNCAP * 4 + LL_IDXCAP * 16)
+#define LL_MAX_N (1 << 24)       // candidate tags pack index | type << 24
 
 typedef const GMEM float4 *cpos_t;
 typedef GMEM float4 *pos_t;
+
+// The blocks that share one system's step: this block's index among them and
+// their count (the whole grid for one system; a slice of it for a batch).
+struct BlockCtx {
+    int bid, nb;
+};
 
 // neighbour-list slot of entry k of particle i (interleaved by tp lanes)
 __device__ __forceinline__ size_t nbl_index(const DP &P, int i, int k) {
@@ -161,67 +169,18 @@ __device__ __forceinline__ void wrap_pbc(float &r, int &dim_img, float L) {
 }
 
 // ---------------------------------------------------------------------------
-// Interactions + integration of particle i by the tp lanes of its group
-// (lane t).  All tp lanes must call it together.
+// Integrator update of particle i once its force (fx, fy, fz) and its
+// interaction scalars are known (NVE.py:59-105 / NVT.py:188-224 /
+// NVT_Langevin.py:342-385), PBC wrap, skin test of the next evaluation and
+// insertion into the next cell grid.  Only 'writer' lanes store.
 // ---------------------------------------------------------------------------
-template <int BASE, int CUT, int INTEG>
-__device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, const PairP *pp, const int i,
-                                              const int t, float acc[GMD_NRED]) {
-    const int tp = P.tp;
+template <int INTEG>
+__device__ __forceinline__ void particle_finish(const DP &P, const StepInfo &I, const int i, const float4 ri,
+                                                float4 v4, const float4 rr, const float fx, const float fy,
+                                                const float fz, const float uu, const float ww, const float ll,
+                                                const bool writer, float acc[GMD_NRED]) {
     const int N = P.N;
-    cpos_t pos = I.cur ? P.pos1 : P.pos0;
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
-    const float4 ri = pos[i];
-    const int cnt = min(P.nbc[i], P.max_nbs);
-    // per-particle state used after the neighbour loop, fetched early
-    float4 v4 = P.vel[i];
-    const float4 rr = P.rref[i];
-    const int ti = __float_as_int(ri.w);
-    const PairP *ppi = pp + ti * P.ntypes;
-    float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
-    // this lane handles entries t, t+tp, ...; UNR gathers in flight, next
-    // batch of list entries prefetched while the current batch is used
-    const GMEM int *nbrow = P.nbl + (size_t)i * tp + t;
-    const size_t rs = (size_t)N * tp;
-    const int nloc = (cnt > t) ? (cnt - t + tp - 1) / tp : 0;
-    constexpr int UNR = 8;
-    int jn[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) jn[u] = (u < nloc) ? nbrow[(size_t)u * rs] : i;
-    for (int m = 0; m < nloc; m += UNR) {
-        float4 rj[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) rj[u] = pos[jn[u]];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) jn[u] = (m + UNR + u < nloc) ? nbrow[(size_t)(m + UNR + u) * rs] : i;
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            float dx, dy, dz;
-            const float d2 = dist_sq_mic(rj[u].x, rj[u].y, rj[u].z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
-            const PairP &q = ppi[__float_as_int(rj[u].w)];
-            if (m + u < nloc && d2 < q.cut2) {
-                float u_, sv, umm;
-                pair_fast<BASE, CUT>(d2, q, u_, sv, umm);
-                fx -= dx * sv;
-                fy -= dy * sv;
-                fz -= dz * sv;
-                ww += d2 * sv;
-                uu += u_;
-                ll += umm - 2.0f * sv;  // (1 - D) s + u'' with D = 3
-            }
-        }
-    }
-    for (int mm = 1; mm < tp; mm <<= 1) {
-        fx += __shfl_xor(fx, mm, 64);
-        fy += __shfl_xor(fy, mm, 64);
-        fz += __shfl_xor(fz, mm, 64);
-        uu += __shfl_xor(uu, mm, 64);
-        ww += __shfl_xor(ww, mm, 64);
-        ll += __shfl_xor(ll, mm, 64);
-    }
-    uu *= 0.5f;
-    ww *= (0.5f / 3.0f);
-    const bool writer = (t == 0);
     if (I.apply_shift) {
 #pragma clang fp contract(off)
         v4.x = v4.x - I.sh0;
@@ -313,171 +272,315 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
     }
 }
 
+// Pair term of the full-list calculator (pair_potential.py:162-178),
+// accumulated as sums over pairs: f -= dr s, W += r^2 s, U += u, lap += u'' - 2 s
+template <int BASE, int CUT>
+__device__ __forceinline__ void pair_accum(const float d2, const float dx, const float dy, const float dz,
+                                           const PairP &q, float &fx, float &fy, float &fz, float &uu, float &ww,
+                                           float &ll, const bool lap) {
+    float u_, sv, umm;
+    pair_fast<BASE, CUT>(d2, q, u_, sv, umm, lap);
+    fx -= dx * sv;
+    fy -= dy * sv;
+    fz -= dz * sv;
+    ww += d2 * sv;
+    uu += u_;
+    if (lap) ll += umm - 2.0f * sv;  // (1 - D) s + u'' with D = 3
+}
+
+// ---------------------------------------------------------------------------
+// Interactions from the neighbour list + integration of particle i by the tp
+// lanes of its group (lane t).  All tp lanes must call it together.
+// ---------------------------------------------------------------------------
+template <int BASE, int CUT, int INTEG>
+__device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, const PairP *pp, const int i,
+                                              const int t, float acc[GMD_NRED]) {
+    const int tp = P.tp;
+    const int N = P.N;
+    cpos_t pos = I.cur ? P.pos1 : P.pos0;
+    const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
+    // the first batch of list entries is requested before the count arrives
+    // (the list is zero-initialised, so entries past the count are valid indices)
+    const GMEM int *nbrow = P.nbl + (size_t)i * tp + t;
+    const size_t rs = (size_t)N * tp;
+    constexpr int UNR = 8;
+    int jn[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) jn[u] = __builtin_nontemporal_load(nbrow + (size_t)u * rs);
+    const float4 ri = pos[i];
+    const int cnt = min(P.nbc[i], P.max_nbs);
+    // per-particle state used after the neighbour loop, fetched early
+    const float4 v4 = P.vel[i];
+    const float4 rr = P.rref[i];
+    const int ti = __float_as_int(ri.w);
+    const PairP *ppi = pp + ti * P.ntypes;
+    const bool lap = P.compute_lap != 0;
+    float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
+    // this lane handles entries t, t+tp, ...; UNR gathers in flight, next
+    // batch of list entries prefetched while the current batch is used
+    const int nloc = (cnt > t) ? (cnt - t + tp - 1) / tp : 0;
+    for (int m = 0; m < nloc; m += UNR) {
+        float4 rj[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) rj[u] = pos[jn[u]];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            jn[u] = (m + UNR + u < nloc) ? __builtin_nontemporal_load(nbrow + (size_t)(m + UNR + u) * rs) : i;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            float dx, dy, dz;
+            const float d2 = dist_sq_mic(rj[u].x, rj[u].y, rj[u].z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
+            const PairP &q = ppi[__float_as_int(rj[u].w)];
+            if (m + u < nloc && d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
+        }
+    }
+    for (int mm = 1; mm < tp; mm <<= 1) {
+        fx += __shfl_xor(fx, mm, 64);
+        fy += __shfl_xor(fy, mm, 64);
+        fz += __shfl_xor(fz, mm, 64);
+        uu += __shfl_xor(uu, mm, 64);
+        ww += __shfl_xor(ww, mm, 64);
+        ll += __shfl_xor(ll, mm, 64);
+    }
+    particle_finish<INTEG>(P, I, i, ri, v4, rr, fx, fy, fz, uu * 0.5f, ww * (0.5f / 3.0f), ll, t == 0, acc);
+}
+
 // ---------------------------------------------------------------------------
 // Steps without a rebuild: tp lanes per particle, grid-stride over particles
 // ---------------------------------------------------------------------------
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_lists(const DP &P, const StepInfo &I, const PairP *pp, float acc[GMD_NRED]) {
+__device__ inline void step_lists(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp,
+                                  float acc[GMD_NRED]) {
     const int t = threadIdx.x & (P.tp - 1);
     const int slot = threadIdx.x >> P.tp_shift;
     const int slots = blockDim.x >> P.tp_shift;
-    for (int i = blockIdx.x * slots + slot; i < P.N; i += gridDim.x * slots)
+    for (int i = B.bid * slots + slot; i < P.N; i += B.nb * slots)
         particle_step<BASE, CUT, INTEG>(P, I, pp, i, t, acc);
 }
 
 // ---------------------------------------------------------------------------
-// Rebuild step, linked lists: one wavefront per cell.
+// Rebuild step, linked lists: one wavefront per cell.  The wavefront stages
+// the members of the 125 stencil cells (cell >= (rc+skin)/2,
+// nblist_linked_lists.py:151-182) in LDS, then for each particle of its cell
+// tests them 64 at a time: accepted neighbours are appended to the list with
+// a ballot (stencil order, then particle index: deterministic) and those
+// inside the pair cutoff contribute to the force at once.  Lane q finally
+// holds the totals of particle q and integrates it.
 // ---------------------------------------------------------------------------
+// Test the staged candidates cand[0, ce) against each particle of the cell:
+// cand holds (x, y, z, index | type << 24) per candidate.
+template <int BASE, int CUT>
+__device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const float4 *cand, const int ce,
+                                          const int np, const int myi, const float4 myr, int &mycount,
+                                          float q6[6]) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
+    const bool lap = P.compute_lap != 0;
+    for (int q = 0; q < np; ++q) {
+        const int iq = __shfl(myi, q, 64);
+        const float rx = __shfl(myr.x, q, 64), ry = __shfl(myr.y, q, 64), rz = __shfl(myr.z, q, 64);
+        const int tq = __float_as_int(__shfl(myr.w, q, 64));
+        const PairP *ppq = pp + tq * P.ntypes;
+        int cntq = __shfl(mycount, q, 64);
+        float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
+        for (int kk = 0; kk < ce; kk += 64) {
+            bool ok = false;
+            int j = -1;
+            if (kk + lane < ce) {
+                const float4 cj = cand[kk + lane];
+                const int tagj = __float_as_int(cj.w);
+                j = tagj & 0xffffff;
+                if (pair_accept(P, iq, j)) {
+                    float dx, dy, dz;
+                    const float d2 = dist_sq_mic(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, dx, dy, dz);
+                    ok = d2 < P.cps2 && !excluded(P, iq, j);
+                    const PairP &qq = ppq[tagj >> 24];
+                    if (ok && d2 < qq.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, qq, fx, fy, fz, uu, ww, ll, lap);
+                }
+            }
+            const unsigned long long mask = __ballot(ok);
+            if (ok) {
+                const int slot = cntq + __popcll(mask & lt_mask);
+                if (slot < P.max_nbs) P.nbl[nbl_index(P, iq, slot)] = j;
+            }
+            cntq += __popcll(mask);
+        }
+        fx = wave_sum_f(fx);
+        fy = wave_sum_f(fy);
+        fz = wave_sum_f(fz);
+        uu = wave_sum_f(uu);
+        ww = wave_sum_f(ww);
+        if (lap) ll = wave_sum_f(ll);
+        if (lane == q) {
+            mycount = cntq;
+            q6[0] += fx;
+            q6[1] += fy;
+            q6[2] += fz;
+            q6[3] += uu;
+            q6[4] += ww;
+            q6[5] += ll;
+        }
+    }
+}
+
+__device__ __forceinline__ void lds_sort_segment(int *a, int b, int e) {
+    for (int x = b + 1; x < e; ++x) {
+        int v = a[x], y = x - 1;
+        while (y >= b && a[y] > v) {
+            a[y + 1] = a[y];
+            --y;
+        }
+        a[y + 1] = v;
+    }
+}
+
+__device__ __forceinline__ int stencil_cell(const DP &P, int cx, int cy, int cz, int sc) {
+    int ix = sc / 25 - 2, iy = (sc / 5) % 5 - 2, iz = sc % 5 - 2;
+    int ox = (cx + ix) % P.ncx; if (ox < 0) ox += P.ncx;
+    int oy = (cy + iy) % P.ncy; if (oy < 0) oy += P.ncy;
+    int oz = (cz + iz) % P.ncz; if (oz < 0) oz += P.ncz;
+    return (ox * P.ncy + oy) * P.ncz + oz;
+}
+
+// stage the members of one stencil cell in LDS at dst (n <= ccap, rows of
+// ccap >= 16 ints, 64-byte aligned): the first 8 in two vector loads
+__device__ __forceinline__ void stage_cell_members(const GMEM int *row, int n, int *dst) {
+    const GMEM int4 *r4 = (const GMEM int4 *)row;
+    const int4 m0 = r4[0], m1 = r4[1];
+    const int v[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < n) dst[k] = v[k];
+    for (int k = 8; k < n; k += 4) {
+        const int4 m = r4[k >> 2];
+        const int w[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (k + t < n) dst[k + t] = w[t];
+    }
+    lds_sort_segment(dst, 0, n);
+}
+
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_ll_rebuild(const DP &P, const StepInfo &I, const PairP *pp, char *lds,
+__device__ inline void step_ll_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, char *lds,
                                        float acc[GMD_NRED]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    char *wl = lds + (size_t)wid * LL_WAVE_LDS;
-    int *own = (int *)wl;
-    int *idx = own + LL_OWNCAP;
-    float4 *cand = (float4 *)(idx + LL_IDXCAP);
+    // per wave: own[LL_OWNCAP] | cand[LL_IDXCAP] float4; the sorted member
+    // indices of a pass are staged at the start of cand, then overwritten by
+    // the candidates themselves
+    int *own = (int *)(lds + (size_t)wid * LL_WAVE_LDS);
+    float4 *cand = (float4 *)(own + LL_OWNCAP);
+    int *idx = (int *)cand;
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
-    const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
-    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const GMEM int *ccount = P.ccount + (size_t)I.cbuf * P.ncells;
     const GMEM int *cmem = P.cmem + (size_t)I.cbuf * P.ncells * P.ccap;
-    const int tp = P.tp;
-    for (int c = blockIdx.x * wpb + wid; c < P.ncells; c += gridDim.x * wpb) {
-        const int np = min(ccount[c], P.ccap);
+    for (int c = B.bid * wpb + wid; c < P.ncells; c += B.nb * wpb) {
+        const int cz = c % P.ncz, cy = (c / P.ncz) % P.ncy, cx = c / (P.ncz * P.ncy);
+        // counts of all 125 stencil cells (lane owns stencil cells lane and
+        // lane + 64) and the unsorted row of the cell itself (stencil cell 62)
+        const int sb = lane + 64;
+        const int nca = stencil_cell(P, cx, cy, cz, lane);
+        const int ncb = (sb < 125) ? stencil_cell(P, cx, cy, cz, sb) : 0;
+        const int na = min(ccount[nca], P.ccap);
+        const int nb = (sb < 125) ? min(ccount[ncb], P.ccap) : 0;
+        const int ownm = (lane < P.ccap) ? cmem[(size_t)c * P.ccap + lane] : 0;
+        const int np = __shfl(na, 62, 64);
         if (np == 0) continue;
-        // centre-cell members, sorted by index (deterministic order)
+        // the cell's own particles in index order: lane q takes the q-th
+        int rank = 0;
+        for (int q = 0; q < np; ++q) rank += (__shfl(ownm, q, 64) < ownm) ? 1 : 0;
         wave_sync_lds();
-        if (lane < np) own[lane] = cmem[(size_t)c * P.ccap + lane];
-        wave_sync_lds();
-        if (lane == 0) {
-            for (int x = 1; x < np; ++x) {
-                int v = own[x], y = x - 1;
-                while (y >= 0 && own[y] > v) {
-                    own[y + 1] = own[y];
-                    --y;
-                }
-                own[y + 1] = v;
-            }
-        }
+        if (lane < np) own[rank] = ownm;
         wave_sync_lds();
         const int myi = (lane < np) ? own[lane] : -1;
-        float4 myr = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (myi >= 0) myr = pos[myi];
-        int mycount = 0;
-        const int cz = c % P.ncz, cy = (c / P.ncz) % P.ncy, cx = c / (P.ncz * P.ncy);
-        // walk the 125 stencil cells (nblist_linked_lists.py:151-163) in passes
-        for (int sc0 = 0; sc0 < 125;) {
-            const int sc = sc0 + lane;
-            int n = 0;
-            size_t base = 0;
-            if (sc < 125) {
-                int ix = sc / 25 - 2, iy = (sc / 5) % 5 - 2, iz = sc % 5 - 2;
-                int ox = (cx + ix) % P.ncx; if (ox < 0) ox += P.ncx;
-                int oy = (cy + iy) % P.ncy; if (oy < 0) oy += P.ncy;
-                int oz = (cz + iz) % P.ncz; if (oz < 0) oz += P.ncz;
-                const int nc = (ox * P.ncy + oy) * P.ncz + oz;
-                n = min(ccount[nc], P.ccap);
-                base = (size_t)nc * P.ccap;
-            }
-            int incl = n;
+        float4 myr = make_float4(0.f, 0.f, 0.f, 0.f), myv = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (myi >= 0) {
+            myr = pos[myi];
+            myv = P.vel[myi];
+        }
+        // stencil-order offsets of every cell's members
+        int ia = na, ib = nb;
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                int tv = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += tv;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int ta = __shfl_up(ia, o, 64), tb = __shfl_up(ib, o, 64);
+            if (lane >= o) {
+                ia += ta;
+                ib += tb;
             }
-            const unsigned long long fit = __ballot(sc < 125 && incl <= LL_IDXCAP);
-            const int ntake = __popcll(fit);  // a prefix: incl is non-decreasing
-            const int M = __shfl(incl, ntake - 1, 64);
-            const int off = incl - n;
+        }
+        const int totA = __shfl(ia, 63, 64);
+        ib += totA;
+        const int M = __shfl(ib, 63, 64);
+        const int offa = ia - na, offb = ib - nb;
+        int mycount = 0;
+        float q6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        // passes over consecutive stencil cells holding at most LL_IDXCAP members
+        // (a cell holds at most ccap <= LL_OWNCAP members, so every pass advances)
+        for (int done = 0; done < M;) {
+            const bool ina = offa >= done && ia - done <= LL_IDXCAP;
+            const bool inb = sb < 125 && offb >= done && ib - done <= LL_IDXCAP;
+            int pe = max(ina ? ia : done, inb ? ib : done);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) pe = max(pe, __shfl_xor(pe, o, 64));
+            const int ce = pe - done;
             wave_sync_lds();
-            if (lane < ntake) {
-                for (int k = 0; k < n; ++k) idx[off + k] = cmem[base + k];
-                for (int x = off + 1; x < off + n; ++x) {  // sort this stencil cell's members
-                    int v = idx[x], y = x - 1;
-                    while (y >= off && idx[y] > v) {
-                        idx[y + 1] = idx[y];
-                        --y;
-                    }
-                    idx[y + 1] = v;
+            if (ina && na > 0) stage_cell_members(cmem + (size_t)nca * P.ccap, na, idx + (offa - done));
+            if (inb && nb > 0) stage_cell_members(cmem + (size_t)ncb * P.ccap, nb, idx + (offb - done));
+            wave_sync_lds();
+            int jj[LL_PER];
+#pragma unroll
+            for (int u = 0; u < LL_PER; ++u) jj[u] = (lane + 64 * u < ce) ? idx[lane + 64 * u] : 0;
+            wave_sync_lds();
+#pragma unroll
+            for (int u = 0; u < LL_PER; ++u) {
+                if (64 * u < ce) {
+                    const float4 r = pos[jj[u]];
+                    const int tag = jj[u] | (__float_as_int(r.w) << 24);
+                    if (lane + 64 * u < ce) cand[lane + 64 * u] = make_float4(r.x, r.y, r.z, __int_as_float(tag));
                 }
             }
             wave_sync_lds();
-            for (int cb = 0; cb < M; cb += LL_POSCAP) {
-                const int ce = min(LL_POSCAP, M - cb);
-                wave_sync_lds();
-                for (int k = lane; k < ce; k += 64) {
-                    const int j = idx[cb + k];
-                    float4 rj = pos[j];
-                    rj.w = __int_as_float(j);
-                    cand[k] = rj;
-                }
-                wave_sync_lds();
-                for (int q = 0; q < np; ++q) {
-                    const int iq = __shfl(myi, q, 64);
-                    const float rx = __shfl(myr.x, q, 64), ry = __shfl(myr.y, q, 64), rz = __shfl(myr.z, q, 64);
-                    int cntq = __shfl(mycount, q, 64);
-                    for (int kk = 0; kk < ce; kk += 64) {
-                        bool ok = false;
-                        int j = -1;
-                        if (kk + lane < ce) {
-                            const float4 cj = cand[kk + lane];
-                            j = __float_as_int(cj.w);
-                            if (pair_accept(P, iq, j)) {
-                                float dx, dy, dz;
-                                const float d2 = dist_sq_mic(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, dx, dy, dz);
-                                ok = d2 < P.cps2 && !excluded(P, iq, j);
-                            }
-                        }
-                        const unsigned long long mask = __ballot(ok);
-                        if (ok) {
-                            const int slot = cntq + __popcll(mask & lt_mask);
-                            if (slot < P.max_nbs) P.nbl[nbl_index(P, iq, slot)] = j;
-                        }
-                        cntq += __popcll(mask);
-                    }
-                    if (lane == q) mycount = cntq;
-                }
-            }
-            sc0 += ntake;
+            cell_test<BASE, CUT>(P, pp, cand, ce, np, myi, myr, mycount, q6);
+            done = pe;
         }
         if (myi >= 0) {
             P.nbc[myi] = mycount;
             P.rref[myi] = myr;
             if (mycount > P.max_nbs) atomicMax(gen_ptr(&P.ctrl->nbflag[1]), mycount);
-        }
-        // the lists just written by this wavefront feed its force evaluation
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const int per = 64 >> P.tp_shift;
-        for (int q0 = 0; q0 < np; q0 += per) {
-            const int q = q0 + (lane >> P.tp_shift);
-            const int iq = __shfl(myi, min(q, 63), 64);
-            if (q < np) particle_step<BASE, CUT, INTEG>(P, I, pp, iq, lane & (tp - 1), acc);
+            particle_finish<INTEG>(P, I, myi, myr, myv, myr, q6[0], q6[1], q6[2], q6[3] * 0.5f,
+                                   q6[4] * (0.5f / 3.0f), q6[5], true, acc);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Rebuild step, N squared (nblist.py:89-143): positions staged through LDS
-// tiles, one thread per particle, appends in ascending j; then the block's
-// particles are evaluated and integrated with tp lanes each.
+// tiles, one thread per particle, neighbours appended in ascending j; pairs
+// inside the cutoff contribute to the force at once, then the particle is
+// integrated.  Every thread of every block runs the same number of rounds.
 // ---------------------------------------------------------------------------
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_nsq_rebuild(const DP &P, const StepInfo &I, const PairP *pp, float4 *tile,
+__device__ inline void step_nsq_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, float4 *tile,
                                         float acc[GMD_NRED]) {
     const int bd = blockDim.x;
-    const int per_round = gridDim.x * bd;
+    const int per_round = B.nb * bd;
     const int rounds = (P.N + per_round - 1) / per_round;
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
+    const bool lap = P.compute_lap != 0;
     for (int rd = 0; rd < rounds; ++rd) {
-        const int i = (rd * gridDim.x + blockIdx.x) * bd + threadIdx.x;
+        const int i = (rd * B.nb + B.bid) * bd + threadIdx.x;
         const bool valid = i < P.N;
-        float4 ri = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (valid) ri = pos[i];
+        float4 ri = make_float4(0.f, 0.f, 0.f, 0.f), vi = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (valid) {
+            ri = pos[i];
+            vi = P.vel[i];
+        }
+        const PairP *ppi = pp + (valid ? __float_as_int(ri.w) : 0) * P.ntypes;
         int cnt = 0;
+        float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
         for (int t0 = 0; t0 < P.N; t0 += bd) {
             __syncthreads();
             const int jl = t0 + threadIdx.x;
@@ -494,6 +597,8 @@ __device__ inline void step_nsq_rebuild(const DP &P, const StepInfo &I, const Pa
                     if (d2 < P.cps2 && !excluded(P, i, j)) {
                         if (cnt < P.max_nbs) P.nbl[nbl_index(P, i, cnt)] = j;
                         ++cnt;
+                        const PairP &q = ppi[__float_as_int(rj.w)];
+                        if (d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
                     }
                 }
             }
@@ -502,42 +607,32 @@ __device__ inline void step_nsq_rebuild(const DP &P, const StepInfo &I, const Pa
             P.nbc[i] = cnt;
             P.rref[i] = ri;
             if (cnt > P.max_nbs) atomicMax(gen_ptr(&P.ctrl->nbflag[1]), cnt);
+            particle_finish<INTEG>(P, I, i, ri, vi, ri, fx, fy, fz, uu * 0.5f, ww * (0.5f / 3.0f), ll, true, acc);
         }
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int t = threadIdx.x & (P.tp - 1);
-    const int slot = threadIdx.x >> P.tp_shift;
-    const int slots = bd >> P.tp_shift;
-    for (int rd = 0; rd < rounds; ++rd)
-        for (int sub = 0; sub < P.tp; ++sub) {
-            const int i = (rd * gridDim.x + blockIdx.x) * bd + sub * slots + slot;
-            if (i < P.N) particle_step<BASE, CUT, INTEG>(P, I, pp, i, t, acc);
-        }
 }
 
 // One full step body (everything but the cross-block tail)
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_body(const DP &P, const StepInfo &I, const PairP *pp, char *build, float acc[GMD_NRED]) {
+__device__ inline void step_body(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, char *build,
+                                 float acc[GMD_NRED]) {
     if (I.rebuild) {
         if (P.nblist_kind == 1)
-            step_ll_rebuild<BASE, CUT, INTEG>(P, I, pp, build, acc);
+            step_ll_rebuild<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
         else
-            step_nsq_rebuild<BASE, CUT, INTEG>(P, I, pp, (float4 *)build, acc);
+            step_nsq_rebuild<BASE, CUT, INTEG>(P, B, I, pp, (float4 *)build, acc);
     } else {
-        step_lists<BASE, CUT, INTEG>(P, I, pp, acc);
+        step_lists<BASE, CUT, INTEG>(P, B, I, pp, acc);
     }
     if (P.nblist_kind == 1 && I.integrate) {
         // clear the cell buffer that step s+2 will fill
         GMEM int *clr = P.ccount + (size_t)I.cbuf_clear * P.ncells;
-        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < P.ncells; c += gridDim.x * blockDim.x) clr[c] = 0;
+        for (int c = B.bid * blockDim.x + threadIdx.x; c < P.ncells; c += B.nb * blockDim.x) clr[c] = 0;
     }
 }
 
 // Block-level reduction of the per-thread sums into partial[block] (fixed order).
-__device__ inline void block_partials(const DP &P, float acc[GMD_NRED], double *red) {
+__device__ inline void block_partials(const DP &P, const BlockCtx &B, float acc[GMD_NRED], double *red) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int c = 0; c < GMD_NRED; ++c) {
@@ -548,7 +643,7 @@ __device__ inline void block_partials(const DP &P, float acc[GMD_NRED], double *
     if (threadIdx.x < GMD_NRED) {
         double sum = 0.0;
         for (int w = 0; w < nw; ++w) sum += red[w * GMD_NRED + threadIdx.x];
-        P.partial[(size_t)blockIdx.x * GMD_NRED + threadIdx.x] = (float)sum;
+        P.partial[(size_t)B.bid * GMD_NRED + threadIdx.x] = (float)sum;
     }
     __syncthreads();
 }
@@ -557,14 +652,14 @@ __device__ inline void block_partials(const DP &P, float acc[GMD_NRED], double *
 // one-thread updates of the reference (NVT.py:226-237 thermostat,
 // momentum_reset.py:166-186 shift, scalar_saver.py:374-408 output row), the
 // rebuild decision of the next step and the step counter.
-__device__ inline void step_tail(const DP &P, const StepInfo &I, double *red) {
+__device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, const int nblocks) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const bool need = I.need_ksum || I.need_mom || I.need_save;
     if (need) {
         double acc[GMD_NRED];
 #pragma unroll
         for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
             for (int c = 0; c < GMD_NRED; ++c) acc[c] += (double)P.partial[(size_t)b * GMD_NRED + c];
         }
@@ -618,7 +713,7 @@ __device__ inline void step_tail(const DP &P, const StepInfo &I, double *red) {
 // Software grid barrier for the cooperative (co-resident) launch.  The last
 // block to arrive runs the tail before releasing the others.
 template <typename Tail>
-__device__ inline void grid_barrier(GMEM Ctrl *C, Tail tail) {
+__device__ inline void grid_barrier(GMEM Ctrl *C, const int nb, Tail tail) {
     __shared__ int s_last;
     __shared__ unsigned s_gen;
     __syncthreads();
@@ -627,7 +722,7 @@ __device__ inline void grid_barrier(GMEM Ctrl *C, Tail tail) {
         // release this block's writes, acquire everybody's when last
         const unsigned arrived =
             __hip_atomic_fetch_add(gen_ptr(&C->bar_count), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (arrived == gridDim.x - 1);
+        s_last = (arrived == (unsigned)nb - 1u);
         s_gen = gen;
     }
     __syncthreads();

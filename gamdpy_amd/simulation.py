@@ -51,6 +51,66 @@ def get_default_compute_plan(configuration):
     return {'pb': pb, 'tp': tp, 'skin': skin, 'UtilizeNIII': False, 'gridsync': gridsync, 'nblist': nblist}
 
 
+def resolve_compute_flags(configuration, runtime_actions):
+    """Simulation.py:146-164: the compute flags asked for by (at most one of)
+    the runtime actions, checked against the configuration's."""
+    compute_flags = None
+    for runtime_action in runtime_actions:
+        if runtime_action.get_compute_flags() is not None:
+            if compute_flags is not None:
+                raise ValueError('Can not handle more than one compute_flags in runtime_actions')
+            compute_flags = runtime_action.get_compute_flags()
+    flags = get_default_compute_flags()
+    if compute_flags is not None:
+        for k in compute_flags:
+            if k in flags:
+                flags[k] = compute_flags[k]
+            else:
+                raise ValueError('Unknown key in compute_flags:%s' % k)
+    for k in flags:
+        if flags[k] and not configuration.compute_flags[k]:
+            raise ValueError('compute_flags["%s]" set for Simulation but not in Configuration' % k)
+    return flags
+
+
+def build_descriptor(conf, pp, integrator, runtime_actions, plan, compute_flags):
+    """Field values of ``gmd_desc`` for one system: what Simulation.__init__
+    gathers from the configuration, interaction, integrator, runtime actions
+    and compute plan (Simulation.py:150-260).  Returns (fields, parameter
+    table, exclusions)."""
+    ip = pp.get_params(conf, plan)
+    table = ip["table"]
+    ntypes = table.shape[0]
+    if conf.N and (np.max(conf.ptype) >= ntypes or np.min(conf.ptype) < 0):
+        raise ValueError(f"particle types must lie in [0, {ntypes}) for this PairPotential")
+    fields = {
+        "N": conf.N, "D": conf.D, "ntypes": ntypes, "box": conf.simbox.lengths,
+        "pair_base": _native.PAIR_BASE[ip["pair_base"]], "cutoff": _native.CUTOFF[ip["cutoff"]],
+        "max_cut": float(ip["max_cut"]),
+        "nblist_kind": _native.NBLIST[ip["nblist_kind"]], "skin": float(ip["skin"]),
+        "max_nbs": int(ip["max_nbs"]), "utilize_niii": int(bool(plan.get('UtilizeNIII', False))),
+        "compute_u": int(compute_flags['U']), "compute_w": int(compute_flags['W']),
+        "compute_lap": int(compute_flags['lapU']), "compute_k": int(compute_flags['K']),
+        "compute_fsq": int(compute_flags['Fsq']),
+        "nscalars": len(conf.scalar_columns),
+        "sid_u": conf.sid.get('U', -1), "sid_w": conf.sid.get('W', -1), "sid_lap": conf.sid.get('lapU', -1),
+        "sid_k": conf.sid.get('K', -1), "sid_fsq": conf.sid.get('Fsq', -1), "sid_m": conf.sid['m'],
+        "pb": int(plan['pb']), "tp": int(plan['tp']), "gridsync": int(bool(plan['gridsync'])),
+        "use_graph": int(bool(plan.get('graph', True))),
+    }
+    fields.update(integrator.get_params(conf, ip))
+    seen = set()
+    for ra in runtime_actions:
+        for k, v in ra.get_params(conf, plan).items():
+            if k in seen:
+                raise NotImplementedError(f"more than one runtime action sets {k}")
+            seen.add(k)
+            fields[k] = v
+    if compute_flags['Ptot'] or compute_flags['stresses']:
+        raise NotImplementedError("Ptot/stresses are not supported by the MI355X path")
+    return fields, table, ip["exclusions"]
+
+
 class Simulation:
     """Run a configuration forward in time with the given interactions, integrator
     and runtime actions (simulation/Simulation.py:15-180)."""
@@ -102,22 +162,7 @@ class Simulation:
         self.memory.attrs['script_name'] = sys.argv[0] if sys.argv else ''
 
         self.runtime_actions = runtime_actions
-        compute_flags = None
-        for runtime_action in self.runtime_actions:
-            if runtime_action.get_compute_flags() is not None:
-                if compute_flags is not None:
-                    raise ValueError('Can not handle more than one compute_flags in runtime_actions')
-                compute_flags = runtime_action.get_compute_flags()
-        self.compute_flags = get_default_compute_flags()
-        if compute_flags is not None:
-            for k in compute_flags:
-                if k in self.compute_flags:
-                    self.compute_flags[k] = compute_flags[k]
-                else:
-                    raise ValueError('Unknown key in compute_flags:%s' % k)
-        for k in self.compute_flags:
-            if self.compute_flags[k] and not configuration.compute_flags[k]:
-                raise ValueError('compute_flags["%s]" set for Simulation but not in Configuration' % k)
+        self.compute_flags = resolve_compute_flags(configuration, runtime_actions)
         for runtime_action in self.runtime_actions:
             runtime_action.setup(configuration=self.configuration, num_timeblocks=num_timeblocks,
                                  steps_per_timeblock=steps_per_timeblock, output=self.memory)
@@ -136,40 +181,8 @@ class Simulation:
         return self.memory
 
     def _descriptor(self):
-        conf = self.configuration
-        plan = self.compute_plan
-        pp = self.interactions[0]
-        ip = pp.get_params(conf, plan)
-        table = ip["table"]
-        ntypes = table.shape[0]
-        if conf.N and (np.max(conf.ptype) >= ntypes or np.min(conf.ptype) < 0):
-            raise ValueError(f"particle types must lie in [0, {ntypes}) for this PairPotential")
-        fields = {
-            "N": conf.N, "D": conf.D, "ntypes": ntypes, "box": conf.simbox.lengths,
-            "pair_base": _native.PAIR_BASE[ip["pair_base"]], "cutoff": _native.CUTOFF[ip["cutoff"]],
-            "max_cut": float(ip["max_cut"]),
-            "nblist_kind": _native.NBLIST[ip["nblist_kind"]], "skin": float(ip["skin"]),
-            "max_nbs": int(ip["max_nbs"]), "utilize_niii": int(bool(plan.get('UtilizeNIII', False))),
-            "compute_u": int(self.compute_flags['U']), "compute_w": int(self.compute_flags['W']),
-            "compute_lap": int(self.compute_flags['lapU']), "compute_k": int(self.compute_flags['K']),
-            "compute_fsq": int(self.compute_flags['Fsq']),
-            "nscalars": len(conf.scalar_columns),
-            "sid_u": conf.sid.get('U', -1), "sid_w": conf.sid.get('W', -1), "sid_lap": conf.sid.get('lapU', -1),
-            "sid_k": conf.sid.get('K', -1), "sid_fsq": conf.sid.get('Fsq', -1), "sid_m": conf.sid['m'],
-            "pb": int(plan['pb']), "tp": int(plan['tp']), "gridsync": int(bool(plan['gridsync'])),
-            "use_graph": int(bool(plan.get('graph', True))),
-        }
-        fields.update(self.integrator.get_params(conf, ip))
-        seen = set()
-        for ra in self.runtime_actions:
-            for k, v in ra.get_params(conf, plan).items():
-                if k in seen:
-                    raise NotImplementedError(f"more than one runtime action sets {k}")
-                seen.add(k)
-                fields[k] = v
-        if self.compute_flags['Ptot'] or self.compute_flags['stresses']:
-            raise NotImplementedError("Ptot/stresses are not supported by the MI355X path")
-        return fields, table, ip["exclusions"]
+        return build_descriptor(self.configuration, self.interactions[0], self.integrator, self.runtime_actions,
+                                self.compute_plan, self.compute_flags)
 
     def _make_context(self):
         if self.ctx is not None:

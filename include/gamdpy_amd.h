@@ -20,11 +20,12 @@
 extern "C" {
 #endif
 
-#define GMD_ABI_VERSION 3
+#define GMD_ABI_VERSION 4
 
 typedef struct gmd_ctx gmd_ctx;
 typedef struct gmd_rdf gmd_rdf;
 typedef struct gmd_batch gmd_batch;
+typedef struct gmd_adj gmd_adj;
 
 /* pair function (interactions/potential_functions/) */
 enum { GMD_LJ_12_6_SIGMA_EPSILON = 0, GMD_LJ_12_6 = 1 };
@@ -150,6 +151,58 @@ int gmd_rdf_compute_ctx(gmd_rdf *rdf, gmd_ctx *ctx, const int *ptype, unsigned l
 /* positions from a host array (N,3) */
 int gmd_rdf_compute_host(gmd_rdf *rdf, const float *r, const int *ptype, const float box[3],
                          unsigned long long *counts);
+
+/* Batches of independent replicas (no counterpart in the reference, which runs
+ * one system per Simulation; this replaces running Simulation.run() once per
+ * replica).  All replicas share pair function, cutoff, integrator, number of
+ * types, tp and pb; N, box, parameters, state point, list kind and runtime
+ * actions are per replica.  blocks[r] = blocks that work on replica r (NULL:
+ * enough for one particle slot per thread).
+ *   GMD_BATCH_PERSISTENT: one cooperative launch per run; the blocks of a
+ *     replica meet at their own barrier each step, replicas never wait for
+ *     each other;
+ *   GMD_BATCH_GRAPH: one launch per step for all replicas, in a hipGraph. */
+enum { GMD_BATCH_PERSISTENT = 0, GMD_BATCH_GRAPH = 1 };
+int gmd_batch_create(const gmd_desc *descs, int nrep, const int *blocks, int driver, gmd_batch **out);
+int gmd_batch_destroy(gmd_batch *batch);
+int gmd_batch_size(gmd_batch *batch, int *nrep, int *grid);
+/* the context of replica rep, for gmd_upload/gmd_download/gmd_read_* and the
+ * RDF; gmd_run, gmd_evaluate and gmd_destroy refuse it */
+int gmd_batch_context(gmd_batch *batch, int rep, gmd_ctx **out);
+/* 'steps' steps of every replica (Simulation.integrate_self per replica) */
+int gmd_batch_run(gmd_batch *batch, float time_zero, int steps);
+int gmd_batch_last_run_ms(gmd_batch *batch, float *ms);
+
+/* Differentiable NVE trajectory (no counterpart in the reference; it adds
+ * gradients of a trajectory observable with respect to the LJ parameters and
+ * the initial positions, for fitting potentials to structural data).
+ * Model: LJ_12_6_sigma_epsilon with gamdpy's shifted-force cutoff
+ * (apply_shifted_force_cutoff.py:29-37, s(r) - s(rc)) at a fixed cut;
+ * NVE leap-frog (NVE.py:59-105); float32 forward, checkpoints every
+ * ckpt_every steps, adjoint by recomputation from the checkpoints.
+ * Observable: H[c][k] = sum over frames (every rdf_every steps, including 0
+ * and, when it is a multiple, nsteps) and unordered pairs of class c of
+ * max(0, exp(-(d - r_k)^2 / (2 w^2)) - exp(-12.5)) for |d - r_k| < 5 w,
+ * r_k = (k + 1/2) rdf_rmax / rdf_bins, w = rdf_width; classes enumerate
+ * type pairs a <= b in row order. */
+typedef struct gmd_adj_desc {
+    int N, ntypes;
+    float box[3];
+    float dt;
+    int nsteps, ckpt_every;
+    int rdf_every, rdf_bins;
+    float rdf_rmax, rdf_width;
+} gmd_adj_desc;
+int gmd_adj_create(const gmd_adj_desc *desc, const int *ptype, const float *mass, gmd_adj **out);
+int gmd_adj_destroy(gmd_adj *adj);
+/* params: ntypes*ntypes*3 float32 (sigma, epsilon, cut), symmetric; r0, v0 (N,3);
+ * H: nclass*bins float64; r_end, v_end (N,3) or NULL */
+int gmd_adj_forward(gmd_adj *adj, const float *params, const float *r0, const float *v0, double *H, float *r_end,
+                    float *v_end);
+/* dLdH: nclass*bins; grad_params: nclass*2 float64 (dL/dsigma, dL/depsilon per
+ * class); grad_r0, grad_v0 (N,3) (grad_v0 may be NULL) */
+int gmd_adj_backward(gmd_adj *adj, const double *dLdH, double *grad_params, float *grad_r0, float *grad_v0);
+int gmd_adj_last_ms(gmd_adj *adj, float *fwd_ms, float *bwd_ms);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,10 @@ def lib():
         L.orc_md_nvt_ll.argtypes = [ctypes.c_int, _f, _f, _f, _i, _i, _f, ctypes.c_int, _f, ctypes.c_float,
                                     ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_longlong, _f, ctypes.c_int, _f, _f, _f, _f]
+        _d = ctypes.POINTER(ctypes.c_double)
+        L.orc_adj_histogram_f64.argtypes = [ctypes.c_int, _d, _d, _d, _i, _d, ctypes.c_int, _d, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, _d, _d]
         _lib = L
     return _lib
 
@@ -79,3 +83,22 @@ def md_nvt_ll(r, v, m, ptype, box, table, max_cut, skin, max_nbs, dt, T, tau, st
     if nreb < 0:
         raise RuntimeError("C oracle: too few cells or neighbour overflow")
     return {"r": r, "v": v, "f": f, "U": U, "W": W, "K": K, "xi": xs[0], "rebuilds": nreb, "img": img}
+
+
+def adj_histogram_f64(r0, v0, mass, ptype, box, params, dt, nsteps, rdf_every, bins, rmax, width):
+    """Float64 trajectory-summed smoothed pair histogram H (ncls, bins) and the
+    final positions; params (nt, nt, 3) = (sigma, epsilon, cut)."""
+    _d = ctypes.POINTER(ctypes.c_double)
+    r0 = np.ascontiguousarray(r0, np.float64)
+    v0 = np.ascontiguousarray(v0, np.float64)
+    m = np.ascontiguousarray(mass, np.float64)
+    pt = np.ascontiguousarray(ptype, np.int32)
+    bx = np.ascontiguousarray(box, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    nt = p.shape[0]
+    H = np.zeros((nt * (nt + 1) // 2, bins), np.float64)
+    r_end = np.zeros_like(r0)
+    lib().orc_adj_histogram_f64(len(r0), _p(r0, _d), _p(v0, _d), _p(m, _d), _p(pt, _i), _p(bx, _d), nt,
+                                _p(p, _d), float(dt), int(nsteps), int(rdf_every), int(bins), float(rmax),
+                                float(width), _p(H, _d), _p(r_end, _d))
+    return H, r_end

@@ -250,3 +250,114 @@ long long orc_md_nvt_ll(int N, float *r, float *v, const float *m, const int *pt
     free(rref);
     return rebuilds;
 }
+
+/*
+ * Differentiable-trajectory observable in float64 (the checker of the
+ * adjoint path; finite differences of it pin the adjoint's gradients).
+ * LJ_12_6_sigma_epsilon with the shifted-force cutoff at a fixed cut
+ * (apply_shifted_force_cutoff.py:29-37, s -= s(rc)): s(r) = 24 eps (2 (sig/r)^12 - (sig/r)^6) / r^2 - s(rc);
+ * leap-frog NVE (NVE.py:59-105) with
+ * the PBC wrap of orthorhombic.py; histogram H[c][k] summed over frames
+ * n % rdf_every == 0 (n = 0..nsteps) and unordered pairs of class c of
+ * max(0, exp(-(d - r_k)^2 / 2w^2) - exp(-12.5)) for |d - r_k| < 5w, r_k = (k + 1/2) rmax / bins.
+ * params: nt*nt*3 (sigma, epsilon, cut); classes a <= b in row order.
+ */
+static inline double mic1d(double d, double L) {
+    double t = 2.0 * d;
+    return t > L ? d - L : (t < -L ? d + L : d);
+}
+
+static int adj_cls(int a, int b, int nt) {
+    if (a > b) {
+        int t = a;
+        a = b;
+        b = t;
+    }
+    return a * nt - a * (a - 1) / 2 + (b - a);
+}
+
+static void adj_hist_frame(int N, const double *r, const int *ptype, const double *box, int nt, int bins,
+                           double rmax, double w, double *H) {
+    const double dr = rmax / bins, rc = rmax + 5.0 * w, floor_ = exp(-12.5);
+    const int ncls = nt * (nt + 1) / 2;
+#pragma omp parallel
+    {
+        double *loc = (double *)calloc((size_t)ncls * bins, sizeof(double));
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < N; ++i)
+            for (int j = i + 1; j < N; ++j) {
+                double dx = mic1d(r[3 * i] - r[3 * j], box[0]);
+                double dy = mic1d(r[3 * i + 1] - r[3 * j + 1], box[1]);
+                double dz = mic1d(r[3 * i + 2] - r[3 * j + 2], box[2]);
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 >= rc * rc) continue;
+                double d = sqrt(d2);
+                int c = adj_cls(ptype[i], ptype[j], nt);
+                for (int k = 0; k < bins; ++k) {
+                    double x = d - (k + 0.5) * dr;
+                    if (fabs(x) >= 5.0 * w) continue;
+                    double kv = exp(-0.5 * x * x / (w * w)) - floor_;
+                    if (kv > 0.0) loc[(size_t)c * bins + k] += kv;
+                }
+            }
+#pragma omp critical
+        for (size_t k = 0; k < (size_t)ncls * bins; ++k) H[k] += loc[k];
+        free(loc);
+    }
+}
+
+int orc_adj_histogram_f64(int N, const double *r0, const double *v0, const double *mass, const int *ptype,
+                          const double *box, int nt, const double *params, double dt, int nsteps, int rdf_every,
+                          int bins, double rmax, double width, double *H, double *r_end) {
+    const int ncls = nt * (nt + 1) / 2;
+    double *r = (double *)malloc(sizeof(double) * 3 * N);
+    double *v = (double *)malloc(sizeof(double) * 3 * N);
+    double *f = (double *)malloc(sizeof(double) * 3 * N);
+    double sc[64];
+    for (int a = 0; a < nt * nt; ++a) {
+        double sig = params[3 * a], eps = params[3 * a + 1], rc = params[3 * a + 2];
+        double x6 = pow(sig / rc, 6.0);
+        sc[a] = 24.0 * eps * (2.0 * x6 * x6 - x6) / (rc * rc);
+    }
+    memcpy(r, r0, sizeof(double) * 3 * N);
+    memcpy(v, v0, sizeof(double) * 3 * N);
+    memset(H, 0, sizeof(double) * ncls * bins);
+    for (int n = 0; n <= nsteps; ++n) {
+        if (n % rdf_every == 0) adj_hist_frame(N, r, ptype, box, nt, bins, rmax, width, H);
+        if (n == nsteps) break;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int i = 0; i < N; ++i) {
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            for (int j = 0; j < N; ++j) {
+                if (j == i) continue;
+                double dx = mic1d(r[3 * i] - r[3 * j], box[0]);
+                double dy = mic1d(r[3 * i + 1] - r[3 * j + 1], box[1]);
+                double dz = mic1d(r[3 * i + 2] - r[3 * j + 2], box[2]);
+                double d2 = dx * dx + dy * dy + dz * dz;
+                const double *p = params + 3 * (ptype[i] * nt + ptype[j]);
+                if (d2 >= p[2] * p[2]) continue;
+                double x = p[0] * p[0] / d2, x3 = x * x * x, x6 = x3 * x3;
+                double s = 24.0 * p[1] * (2.0 * x6 - x3) / d2 - sc[ptype[i] * nt + ptype[j]];
+                fx += s * dx;
+                fy += s * dy;
+                fz += s * dz;
+            }
+            f[3 * i] = fx;
+            f[3 * i + 1] = fy;
+            f[3 * i + 2] = fz;
+        }
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) {
+                v[3 * i + k] += f[3 * i + k] / mass[i] * dt;
+                double x = r[3 * i + k] + v[3 * i + k] * dt;
+                if (2.0 * x > box[k]) x -= box[k];
+                if (2.0 * x < -box[k]) x += box[k];
+                r[3 * i + k] = x;
+            }
+    }
+    if (r_end) memcpy(r_end, r, sizeof(double) * 3 * N);
+    free(r);
+    free(v);
+    free(f);
+    return 0;
+}

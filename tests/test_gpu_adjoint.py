@@ -1,0 +1,142 @@
+"""The hand-written adjoint on the GPU (gmd_adj_* through the C ABI).
+
+* forward histogram vs the float64 C program (oracle/cpu_md.c) on the same
+  start: float32 trajectories of 40 steps stay within 1e-3 of the maximum;
+* gradients (sigma, epsilon per class, initial positions) vs the PyTorch
+  float64 autograd program (oracle/adjoint_ref.py, itself pinned to finite
+  differences of the C program on the CPU): relative error <= 1e-2;
+* the differentiable-MD workload (KA N=4096, rho=1.2, T=0.6, 200 NVE steps,
+  dt=0.004, checkpoints every 20 steps, 200-bin smoothed partial g(r), target
+  from the same run at eps_AB=1.4, sigma_BB=0.9): the six parameter gradients
+  vs central finite differences of the float64 C program, relative error
+  <= 5e-2 (float32 trajectory against float64 over 200 chaotic steps), and
+  the adjoint's cost against the forward's.
+"""
+import numpy as np
+import pytest
+import torch
+
+import gamdpy_amd as gp
+from gamdpy_amd.adjoint import DifferentiableNVE, type_classes
+from oracle import adjoint_ref, c_oracle
+from helpers import KA_SIG, KA_EPS
+
+pytestmark = pytest.mark.gpu
+
+
+def _ka(cells, rho, T, seed=0, N=None):
+    conf = gp.Configuration(D=3)
+    if N is None:
+        conf.make_lattice(gp.unit_cells.FCC, cells=cells, rho=rho)
+    else:
+        conf.make_positions(N=N, rho=rho)
+    conf['m'] = 1.0
+    rng = np.random.default_rng(seed)
+    pt = np.zeros(conf.N, np.int32)
+    pt[rng.permutation(conf.N)[:int(round(0.2 * conf.N))]] = 1
+    conf.ptype = pt
+    conf.randomize_velocities(temperature=T, seed=seed)
+    conf['r'] = conf['r'] + rng.uniform(-0.03, 0.03, conf['r'].shape).astype(np.float32)
+    return conf
+
+
+def _c_hist(conf, sig, eps, cut, traj):
+    p = np.stack([sig, eps, cut], -1)
+    H, _ = c_oracle.adj_histogram_f64(conf['r'].astype(np.float64), conf['v'].astype(np.float64),
+                                      conf['m'].astype(np.float64), conf.ptype, conf.simbox.lengths, p,
+                                      traj['dt'], traj['nsteps'], traj['rdf_every'], traj['bins'], traj['rmax'],
+                                      traj['width'])
+    return H
+
+
+SMALL = dict(dt=0.004, nsteps=40, ckpt_every=10, rdf_every=5, bins=100, rmax=3.0, width=0.03)
+
+
+def _model(conf, traj):
+    cut = 2.5 * np.array(KA_SIG)
+    return DifferentiableNVE(conf, dt=traj['dt'], nsteps=traj['nsteps'], cut=cut, ckpt_every=traj['ckpt_every'],
+                             rdf_every=traj['rdf_every'], bins=traj['bins'], rmax=traj['rmax'],
+                             width=traj['width']), cut
+
+
+def test_forward_histogram_matches_float64():
+    conf = _ka([5, 5, 5], 1.2, 0.8)            # N = 500, L = 7.47 > 2 x 2.5
+    m, cut = _model(conf, SMALL)
+    H = m.histogram(torch.tensor(conf['r']), torch.tensor(KA_SIG, dtype=torch.float64),
+                    torch.tensor(KA_EPS, dtype=torch.float64)).numpy()
+    Hc = _c_hist(conf, np.array(KA_SIG), np.array(KA_EPS), cut, SMALL)
+    print(np.abs(H - Hc).max(), Hc.max())
+    assert np.abs(H - Hc).max() <= 1e-3 * Hc.max()
+
+
+def _loss(g, target):
+    return ((g - target) ** 2).mean()
+
+
+def test_gradients_match_float64_autograd():
+    conf = _ka([5, 5, 5], 1.2, 0.8)
+    m, cut = _model(conf, SMALL)
+    sig_t = np.array(KA_SIG) * np.array([[1.0, 1.0], [1.0, 0.9 / 0.88]])
+    eps_t = np.array(KA_EPS) * np.array([[1.0, 1.4 / 1.5], [1.4 / 1.5, 1.0]])
+    target = torch.tensor(_c_hist(conf, sig_t, eps_t, cut, SMALL) / m.norm)
+    # GPU adjoint
+    r0 = torch.tensor(conf['r'].astype(np.float64), requires_grad=True)
+    sig = torch.tensor(KA_SIG, dtype=torch.float64, requires_grad=True)
+    eps = torch.tensor(KA_EPS, dtype=torch.float64, requires_grad=True)
+    _loss(m.rdf(r0, sig, eps), target).backward()
+    # float64 autograd reference
+    r0r = torch.tensor(conf['r'].astype(np.float64), requires_grad=True)
+    sigr = torch.tensor(KA_SIG, dtype=torch.float64, requires_grad=True)
+    epsr = torch.tensor(KA_EPS, dtype=torch.float64, requires_grad=True)
+    H = adjoint_ref.trajectory_histogram(r0r, torch.tensor(conf['v'], dtype=torch.float64),
+                                         torch.tensor(conf['m'], dtype=torch.float64),
+                                         torch.tensor(conf.ptype, dtype=torch.long), sigr, epsr,
+                                         torch.tensor(cut), torch.tensor(conf.simbox.lengths, dtype=torch.float64),
+                                         SMALL['dt'], SMALL['nsteps'], SMALL['rdf_every'], SMALL['bins'],
+                                         SMALL['rmax'], SMALL['width'])
+    _loss(H / torch.tensor(m.norm), target).backward()
+    for got, ref in ((sig.grad, sigr.grad), (eps.grad, epsr.grad)):
+        got, ref = got.numpy(), ref.numpy()
+        print(got, ref)
+        assert np.abs(got - ref).max() <= 1e-2 * np.abs(ref).max()
+    gr, rr = r0.grad.numpy(), r0r.grad.numpy()
+    err = np.linalg.norm(gr - rr) / np.linalg.norm(rr)
+    print("r0 gradient relative error", err)
+    assert err <= 1e-2
+
+
+@pytest.mark.slow
+def test_differentiable_md_workload_vs_finite_differences():
+    traj = dict(dt=0.004, nsteps=200, ckpt_every=20, rdf_every=10, bins=200, rmax=4.0, width=0.02)
+    conf = _ka(None, 1.2, 0.6, N=4096)         # cubic box L = 15.06
+    m, cut = _model(conf, traj)
+    sig0, eps0 = np.array(KA_SIG), np.array(KA_EPS)
+    sig_t = sig0.copy()
+    sig_t[1, 1] = 0.9
+    eps_t = eps0.copy()
+    eps_t[0, 1] = eps_t[1, 0] = 1.4
+    target = _c_hist(conf, sig_t, eps_t, cut, traj) / m.norm
+    sig = torch.tensor(sig0, requires_grad=True)
+    eps = torch.tensor(eps0, requires_grad=True)
+    r0 = torch.tensor(conf['r'].astype(np.float64), requires_grad=True)
+    loss = _loss(m.rdf(r0, sig, eps), torch.tensor(target))
+    loss.backward()
+    fwd_ms, bwd_ms = m.last_ms()
+    print(f"forward {fwd_ms:.2f} ms, adjoint {bwd_ms:.2f} ms, ratio {bwd_ms / fwd_ms:.2f}")
+
+    def L64(s, e):
+        return float(np.mean((_c_hist(conf, s, e, cut, traj) / m.norm - target) ** 2))
+
+    for (a, b) in type_classes(2):
+        for which, base, g in (("sig", sig0, sig.grad.numpy()), ("eps", eps0, eps.grad.numpy())):
+            h = 1e-4 * base[a, b]
+            p, q = base.copy(), base.copy()
+            p[a, b] = p[b, a] = base[a, b] + h
+            q[a, b] = q[b, a] = base[a, b] - h
+            if which == "sig":
+                fd = (L64(p, eps0) - L64(q, eps0)) / (2 * h)
+            else:
+                fd = (L64(sig0, p) - L64(sig0, q)) / (2 * h)
+            ad = g[a, b] + (g[b, a] if a != b else 0.0)
+            print(which, a, b, ad, fd)
+            assert abs(ad - fd) <= 5e-2 * abs(fd) + 1e-12

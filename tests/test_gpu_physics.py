@@ -1,0 +1,166 @@
+"""Physics of whole simulations on the GPU against the reference's own bands
+and against independent rumd3 data.
+
+* LJ state point bands: the reference's test_LJ_nxyz.py:10-151 (FCC at
+  rho = 0.8442, T0 = 1.44, SF cutoff 2.5, dt = 0.005, MomentumReset(100),
+  ScalarSaver(8) with Fsq and lapU, 2 timeblocks of 4096 steps, averages over
+  the second half of block 1): energy variance per particle, kinetic and
+  configurational temperature, virial/energy correlation R and slope Gamma.
+  These bands are the reference's; they also pin the Langevin integrator,
+  whose noise stream cannot be compared bit for bit.
+* KA at rho = 1.2, T = 2.0: mean potential energy and virial per particle
+  against rumd3's energies_mean.dat of the same state point (N = 4000,
+  shifted-potential cutoff 2.5 sigma_ij), starting from rumd3's own snapshot.
+* KA at rho = 1.2, T = 0.8: partial g(r) against rumd3's 1000-bin table
+  (bin width L/1000 and bin centres; the calculator here uses L/2 / bins and
+  left edges, so 500 bins here match rumd3's first 500 rows shifted by half
+  a bin).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gamdpy_amd as gp
+from helpers import GOLDEN, rumd3_conf, ka_pair_potential, plan
+
+pytestmark = pytest.mark.gpu
+
+
+def _lj(nx, ny, nz, integrator):
+    """test_LJ_nxyz.py:10-75 without the ConfigurationSaver."""
+    configuration = gp.Configuration(D=3, compute_flags={'Fsq': True, 'lapU': True})
+    configuration.make_lattice(gp.unit_cells.FCC, cells=[nx, ny, nz], rho=0.8442)
+    configuration['m'] = 1.0
+    configuration.randomize_velocities(temperature=1.44, seed=0)
+    assert configuration.N == nx * ny * nz * 4
+    compute_plan = gp.get_default_compute_plan(configuration)
+    pairfunc = gp.apply_shifted_force_cutoff(gp.LJ_12_6_sigma_epsilon)
+    pairpot = gp.PairPotential(pairfunc, params=[1.0, 1.0, 2.5], max_num_nbs=1000)
+    dt = 0.005
+    if integrator == 'NVE':
+        integ = gp.integrators.NVE(dt=dt)
+    elif integrator == 'NVT':
+        integ = gp.integrators.NVT(temperature=0.70, tau=0.2, dt=dt)
+    else:
+        integ = gp.integrators.NVT_Langevin(temperature=0.70, alpha=0.1, dt=dt, seed=213)
+    runtime_actions = [gp.MomentumReset(100), gp.ScalarSaver(8, {'Fsq': True, 'lapU': True, 'stresses': False})]
+    sim = gp.Simulation(configuration, pairpot, integ, runtime_actions, num_timeblocks=2,
+                        steps_per_timeblock=1024 * 4, compute_plan=compute_plan, storage='memory', verbose=False)
+    for _ in sim.run_timeblocks():
+        pass
+    columns = ['U', 'W', 'lapU', 'Fsq', 'K']
+    return dict(zip(columns, gp.extract_scalars(sim.output, columns, first_block=1))), configuration.N
+
+
+def _results(df, N, D=3):
+    """test_LJ_nxyz.py:77-101."""
+    U, W, lap, Fsq, K = (np.asarray(df[c], np.float64) for c in ['U', 'W', 'lapU', 'Fsq', 'K'])
+    E = U + K
+    var_e = np.var(E) / N
+    h = len(U) // 2
+    U2, W2, E2 = U[h:], W[h:], E[h:]
+    Tkin = np.mean(2 * K[h:] / D / (N - 1))
+    Tconf = np.mean(Fsq[h:] / lap[h:])
+    dU, dW = U2 - U2.mean(), W2 - W2.mean()
+    R = np.dot(dW, dU) / np.sqrt(np.dot(dW, dW) * np.dot(dU, dU))
+    Gamma = np.dot(dW, dU) / np.dot(dU, dU)
+    return var_e, Tkin, Tconf, R, Gamma
+
+
+CELLS = [(4, 4, 4), (5, 5, 13), (16, 16, 32)]
+
+
+@pytest.mark.parametrize("cells", CELLS, ids=lambda c: "x".join(map(str, c)))
+def test_lj_nve_bands(cells):
+    df, N = _lj(*cells, 'NVE')
+    var_e, Tkin, Tconf, R, Gamma = _results(df, N)
+    print(cells, var_e, Tkin, Tconf, R, Gamma)
+    assert var_e < 1e-4                      # test_LJ_nxyz.py:107-111
+    assert 0.67 < Tkin < 0.72
+    assert 0.67 < Tconf < 0.72
+    assert 0.86 < R < 0.99
+    assert 5.1 < Gamma < 6.6
+
+
+@pytest.mark.parametrize("cells", CELLS, ids=lambda c: "x".join(map(str, c)))
+def test_lj_nvt_bands(cells):
+    df, N = _lj(*cells, 'NVT')
+    var_e, Tkin, Tconf, R, Gamma = _results(df, N)
+    print(cells, var_e, Tkin, Tconf, R, Gamma)
+    assert 0.67 < Tkin < 0.73                # test_LJ_nxyz.py:127-131
+    assert 0.67 < Tconf < 0.73
+    assert 0.91 < R < 1.00
+    assert 4.9 < Gamma < 7.2
+
+
+@pytest.mark.parametrize("cells", [(4, 4, 4), (16, 16, 32)], ids=lambda c: "x".join(map(str, c)))
+def test_lj_langevin_bands(cells):
+    df, N = _lj(*cells, 'NVT_Langevin')
+    var_e, Tkin, Tconf, R, Gamma = _results(df, N)
+    print(cells, var_e, Tkin, Tconf, R, Gamma)
+    assert 0.62 < Tkin < 0.78                # test_LJ_nxyz.py:147-151
+    assert 0.62 < Tconf < 0.78
+    assert 0.85 < R < 1.00
+    assert 4.8 < Gamma < 6.8
+
+
+def _ka_rumd3(T, steps_per_block, blocks, save_every, extra=(), nblist='N squared'):
+    conf, _ = rumd3_conf(0)
+    if T != 2.0:
+        conf.randomize_velocities(temperature=T, seed=1)
+    pp = ka_pair_potential(max_nbs=400)
+    integ = gp.NVT(temperature=T, tau=0.2, dt=0.004)
+    saver = gp.ScalarSaver(save_every)
+    sim = gp.Simulation(conf, pp, integ, [saver, *extra], num_timeblocks=blocks, steps_per_timeblock=steps_per_block,
+                        compute_plan=plan(tp=8, gridsync=False, nblist=nblist, skin=0.3), storage='memory')
+    return conf, sim
+
+
+def test_ka_t2_mean_energies_vs_rumd3():
+    """energies_mean.dat (rumd3, KA N=4000, rho=1.2, T=2.0): pe = -4.6539,
+    W = 14.098 per particle.  Statistical error of 80 time units of N=4000 is
+    a few 1e-3 for pe and ~1e-2 for W; the bands are 5x that."""
+    ref = np.load(os.path.join(GOLDEN, "ka_rumd3_means_rdf.npz"))
+    cols = list(ref['energies_columns'])
+    pe_ref = float(ref['energies_mean'][cols.index('pe')])
+    w_ref = float(ref['energies_mean'][cols.index('W')])
+    conf, sim = _ka_rumd3(2.0, 4096, 6, 16)
+    for _ in sim.run_timeblocks():
+        pass
+    U, W, K = gp.extract_scalars(sim.output, ['U', 'W', 'K'], first_block=1)
+    N = conf.N
+    pe, w = np.mean(U) / N, np.mean(W) / N
+    T = np.mean(2 * K / (3 * (N - 1)))
+    print(f"pe {pe:.4f} (rumd3 {pe_ref:.4f})  W {w:.3f} (rumd3 {w_ref:.3f})  Tkin {T:.4f}")
+    assert abs(T - 2.0) < 0.02
+    assert abs(pe - pe_ref) < 0.02
+    assert abs(w - w_ref) < 0.06
+
+
+def test_ka_t08_partial_rdf_vs_rumd3():
+    """KABLJ_rdf_R1.200_T0.800_rumd3.dat: columns r, gAA, gAB, gBA, gBB."""
+    ref = np.load(os.path.join(GOLDEN, "ka_rumd3_means_rdf.npz"))['rdf_T0800'].astype(np.float64)
+    conf, sim = _ka_rumd3(0.8, 512, 48, 64, nblist='linked lists')
+    calc = gp.CalculatorRadialDistribution(conf, bins=500)
+    for block in sim.run_timeblocks():
+        if block >= 16:                         # 16 x 512 steps of equilibration at T = 0.8
+            calc.update()
+    data = calc.read()
+    bw = data['distances'][1] - data['distances'][0]
+    assert abs(bw - (ref[1, 0] - ref[0, 0])) < 1e-5
+    g = np.mean(data['rdf_ptype'], axis=0)     # (2, 2, bins)
+    r = data['distances'] + bw / 2
+    np.testing.assert_allclose(r, ref[:500, 0], atol=1e-5)
+    sel = (r > 0.7) & (r < 3.5)
+    box5 = np.ones(5) / 5                       # 5-bin running mean on both sides against sampling noise
+    for col, (a, b) in zip((1, 2, 4), ((0, 0), (0, 1), (1, 1))):
+        mine = np.convolve(g[a, b], box5, mode='same')[sel]
+        theirs = np.convolve(ref[:500, col], box5, mode='same')[sel]
+        err = np.max(np.abs(mine - theirs))
+        ipk, jpk = np.argmax(mine), np.argmax(theirs)
+        print(f"g{a}{b}: max |dg| {err:.3f}, peak {mine[ipk]:.3f}@{r[sel][ipk]:.3f} "
+              f"vs {theirs[jpk]:.3f}@{r[sel][jpk]:.3f}")
+        assert abs(ipk - jpk) <= 2
+        assert abs(mine[ipk] - theirs[jpk]) < 0.08 * theirs[jpk]
+        assert err < 0.2 * theirs[jpk]

@@ -1,0 +1,166 @@
+"""Host-side API on the CPU: the parts of the Python layer that run before
+any device call (reference behaviour cited per test)."""
+import math
+
+import numpy as np
+import pytest
+
+import gamdpy_amd as gp
+from gamdpy_amd import _native
+
+
+def test_pair_function_kats_of_the_package():
+    """test_potential_functions.py:9-10 on the package's own functions."""
+    assert tuple(float(x) for x in gp.LJ_12_6(1, [2, 3])) == (5.0, 42.0, 438.0)
+    assert tuple(float(x) for x in gp.LJ_12_6_sigma_epsilon(1, [2, 3])) == (48384.0, 585216.0, 7635456.0)
+
+
+def test_cutoff_wrappers_of_the_package():
+    sp = gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon)
+    sf = gp.apply_shifted_force_cutoff(gp.LJ_12_6_sigma_epsilon)
+    p = [1.0, 1.0, 2.5]
+    u_sp, s_sp, _ = sp(np.float32(2.5) * np.float32(0.999999), p)
+    assert abs(float(u_sp)) < 1e-5                   # u(rc) shifted to 0
+    u_sf, s_sf, _ = sf(np.float32(2.5) * np.float32(0.999999), p)
+    assert abs(float(u_sf)) < 1e-5 and abs(float(s_sf)) < 1e-5
+    assert sp._gmd_cutoff == 'SP' and sf._gmd_cutoff == 'SF'
+
+
+def test_configuration_columns_and_lattice():
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[4, 5, 6], rho=0.8442)
+    assert conf.N == 4 * 4 * 5 * 6
+    assert conf['r'].shape == (conf.N, 3) and conf['r'].dtype == np.float32
+    L = conf.simbox.lengths
+    np.testing.assert_allclose(conf.N / np.prod(L), 0.8442, rtol=1e-5)
+    assert np.all(conf['r'] >= -L / 2) and np.all(conf['r'] < L / 2)
+    conf['m'] = 1.0
+    assert np.all(conf['m'] == 1.0)
+    assert np.all(conf.ptype == 0)
+    with pytest.raises(Exception):
+        conf['no_such_column']
+
+
+def test_velocities_exact_temperature_and_zero_momentum():
+    """Configuration.py:260-281,373-385: per-dimension drift removed, then
+    rescaled to T with D(N-1) degrees of freedom."""
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[5, 5, 5], rho=1.0)
+    conf['m'] = 1.0
+    conf.randomize_velocities(temperature=1.44, seed=7)
+    v = conf['v'].astype(np.float64)
+    assert np.all(np.abs(v.sum(axis=0)) < 1e-3)
+    T = np.sum(v * v) / (3 * (conf.N - 1))
+    assert abs(T - 1.44) < 1e-4
+    conf.set_kinetic_temperature(0.5)
+    v = conf['v'].astype(np.float64)
+    assert abs(np.sum(v * v) / (3 * (conf.N - 1)) - 0.5) < 1e-4
+
+
+def test_make_positions_cubic_box():
+    conf = gp.Configuration(D=3)
+    conf.make_positions(N=65536, rho=1.2)
+    assert conf.N == 65536
+    L = conf.simbox.lengths
+    assert np.allclose(L, L[0]) and abs(L[0] - (65536 / 1.2) ** (1 / 3)) < 1e-4
+    # no two particles share a site
+    r = conf['r'].astype(np.float64)
+    key = np.round((r + L / 2) * 1000).astype(np.int64)
+    assert len(np.unique(key, axis=0)) == conf.N
+
+
+def test_pair_potential_parameters():
+    """pair_potential.py:23-51: scalars promoted to 1x1 tables, max cut."""
+    pp = gp.PairPotential(gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon), params=[1.0, 1.0, 2.5],
+                          max_num_nbs=100)
+    table, max_cut = pp.param_table()
+    assert table.shape == (1, 1, 3) and table.dtype == np.float32
+    assert max_cut == np.float32(2.5)
+    sig = [[1.0, 0.8], [0.8, 0.88]]
+    eps = [[1.0, 1.5], [1.5, 0.5]]
+    pp2 = gp.PairPotential(gp.LJ_12_6_sigma_epsilon, params=[sig, eps, np.array(sig) * 2.5], max_num_nbs=100)
+    table2, max_cut2 = pp2.param_table()
+    assert table2.shape == (2, 2, 3)
+    assert max_cut2 == np.float32(2.5)
+    np.testing.assert_allclose(table2[0, 1], [0.8, 1.5, 2.0], rtol=1e-6)
+
+
+def test_unknown_nblist_raises_value_error():
+    """pair_potential.py:94."""
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[4, 4, 4], rho=0.8442)
+    pp = gp.PairPotential(gp.LJ_12_6_sigma_epsilon, params=[1.0, 1.0, 2.5], max_num_nbs=100)
+    with pytest.raises(ValueError):
+        pp.get_params(conf, {'pb': 64, 'tp': 4, 'skin': 0.5, 'UtilizeNIII': False, 'gridsync': True,
+                             'nblist': 'no such list'})
+
+
+def test_default_compute_plan_shape():
+    """get_default_compute_plan.py:8-89 keys; tp fills 256 CUs x 16 waves."""
+    for n, tp_expected, nbl in [(2048, 16, 'N squared'), (65536, 4, 'linked lists'),
+                                (1048576, 1, 'linked lists')]:
+        conf = gp.Configuration(D=3, N=n)
+        conf.simbox = gp.Orthorhombic(3, [100.0, 100.0, 100.0])
+        p = gp.get_default_compute_plan(conf)
+        assert set(p) >= {'pb', 'tp', 'skin', 'UtilizeNIII', 'gridsync', 'nblist'}
+        assert p['tp'] == tp_expected
+        assert p['pb'] * p['tp'] <= 1024
+        assert p['nblist'] == nbl
+
+
+def test_temperature_functions():
+    """make_function.py:6-47."""
+    c = gp.make_function_constant(0.7)
+    assert c(123.0) == np.float32(0.7)
+    r = gp.make_function_ramp(1.0, 10.0, 2.0, 20.0)
+    assert r(5.0) == np.float32(1.0) and r(25.0) == np.float32(2.0)
+    assert abs(r(15.0) - 1.5) < 1e-6
+    s = gp.make_function_sin(period=8.0, amplitude=0.5, offset=1.0)
+    assert abs(s(2.0) - 1.5) < 1e-6
+    assert c._gmd_schedule[0] == 0 and r._gmd_schedule[0] == 1 and s._gmd_schedule[0] == 2
+
+
+def test_nvt_parameters():
+    """NVT.py:33-37: omega2 = 4 pi^2 / tau^2, degrees = N D - D."""
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[4, 4, 4], rho=1.0)
+    integ = gp.NVT(temperature=0.8, tau=0.2, dt=0.004)
+    d = integ.get_params(conf, ())
+    assert abs(d["omega2"] - 4 * math.pi ** 2 / 0.04) < 1e-2
+    assert d["degrees"] == conf.N * 3 - 3
+
+
+def test_extract_scalars_layout():
+    """extract_scalars.py:30-51: (blocks, saves, columns) flattened per column."""
+    out = gp.output.MemoryOutput()
+    out.attrs['scalar_names'] = ['U', 'W', 'K']
+    data = np.arange(2 * 4 * 3, dtype=np.float32).reshape(2, 4, 3)
+    out.create_dataset('scalars', data=data, shape=data.shape, dtype=np.float32)
+    U, K = gp.extract_scalars(out, ['U', 'K'], first_block=1)
+    np.testing.assert_array_equal(U, data[1:, :, 0].ravel())
+    np.testing.assert_array_equal(K, data[1:, :, 2].ravel())
+
+
+def test_simulation_argument_errors_before_any_device_call():
+    """Simulation.py:97-107: steps/timeblocks checks come first."""
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[4, 4, 4], rho=0.8442)
+    conf['m'] = 1.0
+    pp = gp.PairPotential(gp.LJ_12_6_sigma_epsilon, params=[1.0, 1.0, 2.5], max_num_nbs=100)
+    with pytest.raises(ValueError):
+        gp.Simulation(conf, pp, gp.NVE(dt=0.005), [], storage='memory')
+    with pytest.raises(ValueError):
+        gp.Simulation(conf, pp, gp.NVE(dt=0.005), [], num_timeblocks=2, storage='memory')
+    with pytest.raises(ValueError):
+        gp.Simulation(conf, pp, gp.NVE(dt=0.005), [gp.ScalarSaver(8, {'nonsense': True})], num_timeblocks=2,
+                      steps_per_timeblock=16, storage='memory')
+
+
+def test_native_errors_name_the_missing_device():
+    """Without a GPU the native layer raises (no CPU fallback)."""
+    try:
+        info = _native.device_info()
+    except _native.NativeError as e:
+        assert str(e)
+        return
+    assert info["arch"].startswith("gfx")
