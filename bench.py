@@ -35,6 +35,9 @@ RDF_BINS = 1000
 SIG = [[1.00, 0.80], [0.80, 0.88]]
 EPS = [[1.00, 1.50], [1.50, 0.50]]
 METRIC = "particle-steps/s KA-LJ N=65536; batched-replica & adjoint speedup vs plain"
+PROFILE_STEPS = 200
+# HBM bytes per k_step launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (None until measured)
+TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -74,6 +77,9 @@ def main():
     ap.add_argument("--driver", choices=["gridsync", "graph", "plain"], default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched-replica and adjoint measurements")
+    ap.add_argument("--replicas", type=int, default=128, help="replicas of the batched measurement")
+    ap.add_argument("--replica-steps", type=int, default=1000, help="steps of the batched measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -146,6 +152,12 @@ def main():
             rdf_count += 1
     elapsed = time.perf_counter() - t0
     rebuilds = int(ctx.nbflag()[2] - nb0)
+    # per-kernel durations with hipEvents around every launch, on the
+    # library's stream, over the next PROFILE_STEPS steps of the same run
+    prof = None
+    if not plan['gridsync']:
+        k_ms, t_ms = ctx.profile_steps(np.float32(done * DT), PROFILE_STEPS)
+        prof = {"k_step_us": k_ms * 1e3, "k_tail_us": t_ms * 1e3, "steps": PROFILE_STEPS}
     conf.copy_to_host()
     mean_nb, max_nb = list_length_stats(conf, pp)
 
@@ -155,15 +167,19 @@ def main():
     # B_ps = 4 * mean list length + 92 (neighbour indices + per-particle state)
     n_list = mean_nb
     B_ps = 4.0 * n_list + 92.0
-    step_ms = [ms / k for ms, k in kernel_ms]
-    kernel_ms_per_step = float(np.mean(step_ms))
-    achieved = B_ps * N / (kernel_ms_per_step * 1e-3) / 1e9
+    kernel_ms_per_step = sum(ms for ms, _ in kernel_ms) / sum(k for _, k in kernel_ms)
+    k_step_ms = prof["k_step_us"] * 1e-3 if prof else kernel_ms_per_step
+    achieved = B_ps * N / (k_step_ms * 1e-3) / 1e9
     log(f"[bench] {steps} steps in {elapsed:.3f}s: {tps:.1f} TPS, {value:.3e} particle-steps/s; "
         f"step kernel {kernel_ms_per_step * 1e3:.2f} us/step; rebuilds {rebuilds}; mean list {mean_nb:.1f}")
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
+    extras = {}
+    if not args.no_extras:
+        extras["batched_replicas"] = batched_speedup(args.replicas, args.replica_steps)
+        extras["adjoint"] = adjoint_cost()
 
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
@@ -180,13 +196,99 @@ def main():
         "rdf_updates_timed": rdf_count,
         "nblist_rebuilds_timed": rebuilds,
         "mean_list_length": mean_nb,
+        "kernel_profile": prof,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_particle_step": B_ps,
-                     "kernel": "k_persistent<LJ_sigma_epsilon,SP,NVT>" if plan['gridsync'] else "k_step<...>"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES,
+                     "bytes_per_particle_step": B_ps, "units_per_launch": N,
+                     "kernel": "k_persistent<LJ_sigma_epsilon,SP,NVT>" if plan['gridsync']
+                     else "k_step<LJ_sigma_epsilon,SP,NVT>"},
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
+
+
+def _ka_replica(gp, N, rho, T, seed):
+    conf = gp.Configuration(D=3)
+    conf.make_positions(N=N, rho=rho)
+    conf['m'] = 1.0
+    rng = np.random.default_rng(seed)
+    pt = np.zeros(conf.N, np.int32)
+    pt[rng.permutation(conf.N)[:int(round(0.2 * conf.N))]] = 1
+    conf.ptype = pt
+    conf.randomize_velocities(temperature=T, seed=seed)
+    func = gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon)
+    pp = gp.PairPotential(func, params=[SIG, EPS, np.array(SIG) * 2.5], max_num_nbs=512)
+    return conf, pp
+
+
+def batched_speedup(n_rep, steps, seed=0):
+    """Independent KA-LJ replicas (N uniform in 1000-8000, rho in [1.1, 1.3],
+    T in [0.45, 2.0], NVT) advanced by one batch versus the same replicas run
+    one after another through the plain path (each with its default plan).
+    Aggregate particle-steps/s of both, timed on the GPU (hipEvents) over
+    'steps' steps after 200 untimed ones."""
+    import gamdpy_amd as gp
+    rng = np.random.default_rng(seed)
+    specs = [(int(rng.integers(1000, 8001)), float(rng.uniform(1.1, 1.3)), float(rng.uniform(0.45, 2.0)))
+             for _ in range(n_rep)]
+    ntot = sum(n for n, _, _ in specs)
+    # sequential plain runs
+    seq_ms = 0.0
+    for k, (n, rho, T) in enumerate(specs):
+        conf, pp = _ka_replica(gp, n, rho, T, seed + k)
+        sim = gp.Simulation(conf, pp, gp.NVT(temperature=T, tau=0.2, dt=DT), [], num_timeblocks=1,
+                            steps_per_timeblock=steps, storage='memory')
+        sim.integrate_self(0.0, 200)
+        sim.integrate_self(0.0, steps)
+        seq_ms += sim.ctx.last_run_ms()
+        sim.ctx.close()
+    reps = []
+    for k, (n, rho, T) in enumerate(specs):
+        conf, pp = _ka_replica(gp, n, rho, T, seed + k)
+        reps.append(gp.Replica(conf, pp, gp.NVT(temperature=T, tau=0.2, dt=DT)))
+    batch = gp.BatchedSimulation(reps, num_timeblocks=1, steps_per_timeblock=steps, tp=4, skin=0.5)
+    plan = batch.autotune(steps=100, tps=(2, 4), fractions=(1.0, 2.0))
+    batch.integrate_self(0.0, 200)
+    batch.integrate_self(0.0, steps)
+    b_ms = batch.batch.last_run_ms()
+    seq_rate = ntot * steps / (seq_ms * 1e-3)
+    b_rate = ntot * steps / (b_ms * 1e-3)
+    log(f"[bench] batched: {n_rep} replicas, {ntot} particles, {steps} steps: batch {b_ms:.1f} ms "
+        f"({b_rate:.3e} p-s/s, {plan['driver']} tp={plan['tp']}) vs sequential {seq_ms:.1f} ms ({seq_rate:.3e})")
+    return {"replicas": n_rep, "particles": ntot, "steps": steps, "batch_ms": b_ms, "sequential_ms": seq_ms,
+            "batch_particle_steps_per_s": b_rate, "sequential_particle_steps_per_s": seq_rate,
+            "speedup": seq_ms / b_ms, "plan": {"driver": plan['driver'], "tp": plan['tp']}}
+
+
+def adjoint_cost():
+    """Differentiable NVE of KA-LJ N=4096 (FCC 8x8x16, rho 1.2, T 0.6), 200
+    steps, checkpoints every 20, 200-bin smoothed partial g(r): GPU time of
+    the forward pass and of the adjoint pass."""
+    import torch
+    import gamdpy_amd as gp
+    from gamdpy_amd.adjoint import DifferentiableNVE
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[8, 8, 16], rho=RHO)
+    conf['m'] = 1.0
+    rng = np.random.default_rng(0)
+    pt = np.zeros(conf.N, np.int32)
+    pt[rng.permutation(conf.N)[:int(round(0.2 * conf.N))]] = 1
+    conf.ptype = pt
+    conf.randomize_velocities(temperature=0.6, seed=0)
+    m = DifferentiableNVE(conf, dt=DT, nsteps=200, cut=2.5 * np.array(SIG), ckpt_every=20, rdf_every=10,
+                          bins=200, rmax=4.0, width=0.02)
+    out = []
+    for _ in range(3):
+        sig = torch.tensor(SIG, dtype=torch.float64, requires_grad=True)
+        eps = torch.tensor(EPS, dtype=torch.float64, requires_grad=True)
+        r0 = torch.tensor(conf['r'].astype(np.float64), requires_grad=True)
+        g = m.rdf(r0, sig, eps)
+        (g ** 2).mean().backward()
+        out.append(m.last_ms())
+    f_ms, b_ms = out[-1]
+    log(f"[bench] adjoint: forward {f_ms:.2f} ms, backward {b_ms:.2f} ms, ratio {b_ms / f_ms:.2f}")
+    return {"N": conf.N, "steps": 200, "forward_ms": f_ms, "backward_ms": b_ms, "cost_ratio": b_ms / f_ms}
 
 
 def cpu_baseline(budget_s):

@@ -92,6 +92,7 @@ _SIGNATURES = {
     "gmd_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]),
     "gmd_evaluate": (ctypes.c_int, [ctypes.c_void_p]),
     "gmd_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "gmd_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_int, c_float_p, c_float_p]),
     "gmd_read_nbflag": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "gmd_read_nblist": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "gmd_force_rebuild": (ctypes.c_int, [ctypes.c_void_p]),
@@ -281,6 +282,14 @@ class Context:
         x = ctypes.c_float()
         check(self.lib.gmd_last_run_ms(self.h, ctypes.byref(x)), "gmd_last_run_ms")
         return float(x.value)
+
+    def profile_steps(self, time_zero, steps):
+        """Run ``steps`` steps with hipEvents around every kernel; returns the mean
+        (step kernel ms, step-tail kernel ms)."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        check(self.lib.gmd_profile_steps(self.h, ctypes.c_float(np.float32(time_zero)), int(steps), ctypes.byref(a),
+                                         ctypes.byref(b)), "gmd_profile_steps")
+        return float(a.value), float(b.value)
 
     def nbflag(self):
         a = np.zeros(3, np.int32)

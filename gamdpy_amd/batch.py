@@ -120,9 +120,23 @@ class BatchedSimulation:
             plan = replica_plan(r, tp, skin)
             specs.append(build_descriptor(r.configuration, r.interaction, r.integrator, r.runtime_actions, plan,
                                           r.compute_flags))
-        if blocks is None:
-            blocks = self.default_blocks(tp, skin, driver)
-        self.batch = _native.Batch(specs, blocks=blocks, driver=driver)
+        if blocks is None and driver == 'persistent':
+            # largest budget the device holds co-resident: two blocks per compute
+            # unit, else one
+            err = None
+            for per_cu in (2, 1):
+                try:
+                    blocks = self.default_blocks(tp, skin, driver, per_cu)
+                    self.batch = _native.Batch(specs, blocks=blocks, driver=driver)
+                    break
+                except _native.NativeError as e:
+                    err = e
+            else:
+                raise err
+        else:
+            if blocks is None:
+                blocks = self.default_blocks(tp, skin, driver)
+            self.batch = _native.Batch(specs, blocks=blocks, driver=driver)
         self.blocks = None if blocks is None else np.asarray(blocks)
         self.tp, self.skin, self.driver = tp, skin, driver
         for r, ctx in zip(self.replicas, self.batch.contexts):
@@ -132,16 +146,16 @@ class BatchedSimulation:
             r.configuration.copy_to_device()
             r.integrator.upload_state(ctx)
 
-    def default_blocks(self, tp, skin, driver):
-        """Persistent: blocks in proportion to work within two blocks per compute
-        unit (co-resident at 256 threads); graph: one particle slot per thread."""
+    def default_blocks(self, tp, skin, driver, per_cu=2):
+        """Persistent: blocks in proportion to work within per_cu blocks per
+        compute unit; graph: one particle slot per thread."""
         if driver == 'graph':
             return None
         try:
             cus = _native.device_info()["compute_units"]
         except Exception:
             cus = 256
-        total = max(len(self.replicas), 2 * cus)
+        total = max(len(self.replicas), per_cu * cus)
         return assign_blocks(self.replicas, tp, skin, total)
 
     # ------------------------------------------------------------------

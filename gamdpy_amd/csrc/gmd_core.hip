@@ -18,16 +18,27 @@
 // ============================================================================
 // Every step of a run inside one launch; the blocks of one system meet at a
 // software barrier after each step, and the last to arrive runs the tail.
+// a pointer that is the same in every lane, made visibly uniform (loads
+// through it become scalar loads)
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
 template <int BASE, int CUT, int INTEG>
-__device__ inline void persistent_body(const DP &P, const BlockCtx &B) {
+__device__ __forceinline__ void persistent_body(const DP *Pp, const BlockCtx &B) {
     extern __shared__ __align__(16) char lds[];
-    PairP *pp = (PairP *)(lds + P.lds_pp);
-    double *red = (double *)(lds + P.lds_red);
-    char *build = lds + P.lds_build;
-    load_pair_table(P, pp);
+    PairP *pp = (PairP *)(lds + Pp->lds_pp);
+    double *red = (double *)(lds + Pp->lds_red);
+    char *build = lds + Pp->lds_build;
+    load_pair_table(*Pp, pp);
     __syncthreads();
-    const int nsteps = P.ctrl->nsteps;
+    const int nsteps = Pp->ctrl->nsteps;
     for (int s = 0; s <= nsteps; ++s) {
+        const DP &P = *Pp;
         const bool rebuild = ld_flag(P.rebuild_flag + s) != 0;
         const StepInfo I = step_info(P, s, INTEG, rebuild);
         float acc[GMD_NRED];
@@ -42,7 +53,7 @@ __device__ inline void persistent_body(const DP &P, const BlockCtx &B) {
 // One step; the step index and the rebuild decision come from the control
 // block that the previous step's tail wrote.
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_once(const DP &P, const BlockCtx &B) {
+__device__ __forceinline__ void step_once(const DP &P, const BlockCtx &B) {
     extern __shared__ __align__(16) char lds[];
     const int s = P.ctrl->step;
     if (s > P.ctrl->nsteps) return;
@@ -62,7 +73,7 @@ __device__ inline void step_once(const DP &P, const BlockCtx &B) {
 
 template <int BASE, int CUT, int INTEG>
 __global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
-    persistent_body<BASE, CUT, INTEG>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
+    persistent_body<BASE, CUT, INTEG>(Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // One step per launch.  The cross-block work (sums, thermostat, momentum
@@ -71,7 +82,7 @@ __global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
 // each a write-back of the XCD's L2 under the blocks still running, which
 // costs more than one extra launch.
 template <int BASE, int CUT, int INTEG>
-__global__ void __launch_bounds__(256) k_step(const DP *__restrict__ Pp) {
+__global__ void __launch_bounds__(256, 4) k_step(const DP *__restrict__ Pp) {
     step_once<BASE, CUT, INTEG>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
 }
 
@@ -85,15 +96,19 @@ struct BatchBlk {
 template <int BASE, int CUT, int INTEG>
 __global__ void __launch_bounds__(256) k_batch_persistent(const DP *const *Ps, const BatchBlk *blks) {
     const BatchBlk a = blks[blockIdx.x];
-    if (a.rep < 0) return;
-    persistent_body<BASE, CUT, INTEG>(*Ps[a.rep], BlockCtx{a.bid, a.nb});
+    const int rep = __builtin_amdgcn_readfirstlane(a.rep);
+    if (rep < 0) return;
+    const BlockCtx B = {__builtin_amdgcn_readfirstlane(a.bid), __builtin_amdgcn_readfirstlane(a.nb)};
+    persistent_body<BASE, CUT, INTEG>(uniform_ptr(Ps[rep]), B);
 }
 
 template <int BASE, int CUT, int INTEG>
 __global__ void __launch_bounds__(256) k_batch_step(const DP *const *Ps, const BatchBlk *blks) {
     const BatchBlk a = blks[blockIdx.x];
-    if (a.rep < 0) return;
-    step_once<BASE, CUT, INTEG>(*Ps[a.rep], BlockCtx{a.bid, a.nb});
+    const int rep = __builtin_amdgcn_readfirstlane(a.rep);
+    if (rep < 0) return;
+    const BlockCtx B = {__builtin_amdgcn_readfirstlane(a.bid), __builtin_amdgcn_readfirstlane(a.nb)};
+    step_once<BASE, CUT, INTEG>(*uniform_ptr(Ps[rep]), B);
 }
 
 // The cross-block tail of a step as its own one-block kernel (the kernel
@@ -109,7 +124,7 @@ __global__ void __launch_bounds__(256) k_tail(const DP *__restrict__ Pp, int int
 
 // the tails of all replicas of a batch: block r finishes replica r's step
 __global__ void __launch_bounds__(256) k_batch_tail(const DP *const *Ps, int integrator) {
-    const DP &P = *Ps[blockIdx.x];
+    const DP &P = *uniform_ptr(Ps[blockIdx.x]);
     __shared__ double red[4 * GMD_NRED];
     const int s = P.ctrl->step;
     if (s > P.ctrl->nsteps) return;
@@ -907,6 +922,51 @@ int gmd_evaluate(gmd_ctx *c) {
     return run_impl(c, 0.0f, 0);
 }
 
+int gmd_profile_steps(gmd_ctx *c, float time_zero, int steps, float *step_ms, float *tail_ms) {
+    if (!c || !step_ms || !tail_ms) return fail("gmd_profile_steps: null argument");
+    if (!c->own_stream) return fail("gmd_profile_steps: this context is a replica of a batch");
+    if (c->d.gridsync) return fail("gmd_profile_steps: the persistent driver runs every step in one launch");
+    if (c->d.integrator == GMD_INTEGRATOR_NONE && steps > 0) return fail("gmd_profile_steps: no integrator");
+    if (run_prepare(c, time_zero, steps)) return -1;
+    hipStream_t st = c->stream;
+    const DP *dP = c->d_P;
+    const int n = steps + 1;
+    std::vector<hipEvent_t> ev((size_t)3 * n, nullptr);
+    int rc = 0;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            rc = fail("gmd_profile_steps: hipEventCreate failed");
+            break;
+        }
+    if (rc == 0) {
+        for (int s = 0; s < n; ++s) {
+            hipEventRecord(ev[3 * s], st);
+            hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
+            hipEventRecord(ev[3 * s + 1], st);
+            hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, dP, c->d.integrator);
+            hipEventRecord(ev[3 * s + 2], st);
+        }
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            rc = fail("gmd_profile_steps: launch failed");
+    }
+    if (rc == 0) {
+        double a = 0.0, b = 0.0;
+        for (int s = 0; s < n; ++s) {
+            float x = 0.f, y = 0.f;
+            hipEventElapsedTime(&x, ev[3 * s], ev[3 * s + 1]);
+            hipEventElapsedTime(&y, ev[3 * s + 1], ev[3 * s + 2]);
+            a += x;
+            b += y;
+        }
+        *step_ms = (float)(a / n);
+        *tail_ms = (float)(b / n);
+    }
+    for (auto &e : ev)
+        if (e) hipEventDestroy(e);
+    if (rc) return rc;
+    return run_complete(c, steps);
+}
+
 int gmd_last_run_ms(gmd_ctx *c, float *ms) {
     if (!c || !ms) return fail("gmd_last_run_ms: null argument");
     *ms = c->last_ms;
@@ -1168,26 +1228,39 @@ static int batch_free(gmd_batch *b) {
 }
 
 // Grid layout of a batch.  Consecutive workgroups of a launch go to the
-// device's 8 XCDs in turn, so grid slot x*8 + k runs on XCD k.  Replicas are
-// packed into the XCDs greedily by block count (largest first) and each
-// replica's blocks are placed on slots of one XCD: a replica's positions and
-// lists then live in one XCD's L2.  Slots left over in a short XCD are idle
-// blocks (rep = -1) that return at once.
+// device's 8 XCDs in turn, so grid slot k*8 + x runs on XCD x.  The blocks
+// are packed into 8 bins of T = ceil(total/8) slots, largest replica first,
+// each into the fullest bin that still holds it whole (its positions and
+// lists then live in one XCD's L2); a replica that fits no bin whole is
+// spread over the emptiest bins.  At most 7 slots stay idle (rep = -1).
 static void batch_layout(const std::vector<int> &nb, std::vector<BatchBlk> &out) {
     const int NX = 8;
+    long long total = 0;
+    for (int n : nb) total += n;
+    const long long T = (total + NX - 1) / NX;
     std::vector<int> order(nb.size());
     for (size_t r = 0; r < nb.size(); ++r) order[r] = (int)r;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return nb[a] > nb[b2]; });
     std::vector<std::vector<BatchBlk>> bins(NX);
-    std::vector<long long> load(NX, 0);
     for (int r : order) {
-        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        for (int k = 0; k < nb[r]; ++k) bins[x].push_back(BatchBlk{r, k, nb[r], 0});
-        load[x] += nb[r];
+        int best = -1;
+        for (int x = 0; x < NX; ++x) {
+            const long long room = T - (long long)bins[x].size();
+            if (room >= nb[r] && (best < 0 || room < T - (long long)bins[best].size())) best = x;
+        }
+        if (best >= 0) {
+            for (int k = 0; k < nb[r]; ++k) bins[best].push_back(BatchBlk{r, k, nb[r], 0});
+            continue;
+        }
+        int k = 0;
+        while (k < nb[r]) {
+            int x = 0;
+            for (int y = 1; y < NX; ++y)
+                if (bins[y].size() < bins[x].size()) x = y;
+            while (k < nb[r] && (long long)bins[x].size() < T) bins[x].push_back(BatchBlk{r, k++, nb[r], 0});
+        }
     }
-    size_t len = 0;
-    for (auto &v : bins) len = std::max(len, v.size());
-    out.assign(len * NX, BatchBlk{-1, 0, 0, 0});
+    out.assign((size_t)T * NX, BatchBlk{-1, 0, 0, 0});
     for (int x = 0; x < NX; ++x)
         for (size_t k = 0; k < bins[x].size(); ++k) out[k * NX + x] = bins[x][k];
 }

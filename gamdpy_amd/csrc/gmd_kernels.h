@@ -125,7 +125,7 @@ struct StepInfo {
     long long lstep;
 };
 
-__device__ inline StepInfo step_info(const DP &P, int s, int integrator, bool rebuild) {
+__device__ __forceinline__ StepInfo step_info(const DP &P, int s, int integrator, bool rebuild) {
     StepInfo I;
     const GMEM Ctrl *C = P.ctrl;
     I.s = s;
@@ -309,9 +309,6 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
     for (int u = 0; u < UNR; ++u) jn[u] = __builtin_nontemporal_load(nbrow + (size_t)u * rs);
     const float4 ri = pos[i];
     const int cnt = min(P.nbc[i], P.max_nbs);
-    // per-particle state used after the neighbour loop, fetched early
-    const float4 v4 = P.vel[i];
-    const float4 rr = P.rref[i];
     const int ti = __float_as_int(ri.w);
     const PairP *ppi = pp + ti * P.ntypes;
     const bool lap = P.compute_lap != 0;
@@ -342,6 +339,8 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
         ww += __shfl_xor(ww, mm, 64);
         ll += __shfl_xor(ll, mm, 64);
     }
+    const float4 v4 = P.vel[i];
+    const float4 rr = P.rref[i];
     particle_finish<INTEG>(P, I, i, ri, v4, rr, fx, fy, fz, uu * 0.5f, ww * (0.5f / 3.0f), ll, t == 0, acc);
 }
 
@@ -349,7 +348,7 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
 // Steps without a rebuild: tp lanes per particle, grid-stride over particles
 // ---------------------------------------------------------------------------
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_lists(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp,
+__device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp,
                                   float acc[GMD_NRED]) {
     const int t = threadIdx.x & (P.tp - 1);
     const int slot = threadIdx.x >> P.tp_shift;
@@ -463,7 +462,7 @@ __device__ __forceinline__ void stage_cell_members(const GMEM int *row, int n, i
 }
 
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_ll_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, char *lds,
+__device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, char *lds,
                                        float acc[GMD_NRED]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
@@ -562,7 +561,7 @@ __device__ inline void step_ll_rebuild(const DP &P, const BlockCtx &B, const Ste
 // integrated.  Every thread of every block runs the same number of rounds.
 // ---------------------------------------------------------------------------
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_nsq_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, float4 *tile,
+__device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, float4 *tile,
                                         float acc[GMD_NRED]) {
     const int bd = blockDim.x;
     const int per_round = B.nb * bd;
@@ -614,7 +613,7 @@ __device__ inline void step_nsq_rebuild(const DP &P, const BlockCtx &B, const St
 
 // One full step body (everything but the cross-block tail)
 template <int BASE, int CUT, int INTEG>
-__device__ inline void step_body(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, char *build,
+__device__ __forceinline__ void step_body(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, char *build,
                                  float acc[GMD_NRED]) {
     if (I.rebuild) {
         if (P.nblist_kind == 1)
@@ -632,7 +631,7 @@ __device__ inline void step_body(const DP &P, const BlockCtx &B, const StepInfo 
 }
 
 // Block-level reduction of the per-thread sums into partial[block] (fixed order).
-__device__ inline void block_partials(const DP &P, const BlockCtx &B, float acc[GMD_NRED], double *red) {
+__device__ __forceinline__ void block_partials(const DP &P, const BlockCtx &B, float acc[GMD_NRED], double *red) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int c = 0; c < GMD_NRED; ++c) {

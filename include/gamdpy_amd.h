@@ -126,6 +126,10 @@ int gmd_run(gmd_ctx *ctx, float time_zero, int steps);
 int gmd_evaluate(gmd_ctx *ctx);
 /* GPU milliseconds of the last gmd_run (hipEvents around the launch) */
 int gmd_last_run_ms(gmd_ctx *ctx, float *ms);
+/* gmd_run with one launch per kernel and hipEvents around each: the mean
+ * duration of the step kernel and of the step-tail kernel over the steps+1
+ * evaluations (a measurement aid; no counterpart in the reference) */
+int gmd_profile_steps(gmd_ctx *ctx, float time_zero, int steps, float *step_ms, float *tail_ms);
 
 /* NbList.d_nbflag (nblist.py:9): [update pending, overflow count, #rebuilds] */
 int gmd_read_nbflag(gmd_ctx *ctx, int nbflag[3]);
