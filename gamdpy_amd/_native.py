@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgamdpy_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gamdpy_amd.h")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # enums of gamdpy_amd.h
 PAIR_BASE = {"LJ_12_6_sigma_epsilon": 0, "LJ_12_6": 1}
@@ -96,6 +96,7 @@ _SIGNATURES = {
     "gmd_read_nbflag": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "gmd_read_nblist": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "gmd_force_rebuild": (ctypes.c_int, [ctypes.c_void_p]),
+    "gmd_set_reorder_interval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gmd_get_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "gmd_set_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "gmd_read_scalar_output": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
@@ -300,6 +301,9 @@ class Context:
         a = np.zeros((self.N, self.max_nbs + 1), np.int32)
         check(self.lib.gmd_read_nblist(self.h, iptr(a)), "gmd_read_nblist")
         return a
+
+    def set_reorder_interval(self, steps):
+        check(self.lib.gmd_set_reorder_interval(self.h, int(steps)), "gmd_set_reorder_interval")
 
     def force_rebuild(self):
         check(self.lib.gmd_force_rebuild(self.h), "gmd_force_rebuild")

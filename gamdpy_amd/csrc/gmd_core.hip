@@ -181,6 +181,8 @@ struct RdfP {
     int N, nt, bins, nclass, tile, ntiles;
     long long npairs;
     float L0, L1, L2, inv_bw;
+    const GMEM float4 *box;    // per tile: (centre, -), (half extent, -); pairs of tiles farther apart than skip2 are skipped
+    float skip2;
     cpos_t pos;
     const GMEM int *ptype;     // nullptr: type from pos.w
     const GMEM float2 *thr2;   // (thr[k], thr[k+1]) k = 0..bins-1; thr = bin thresholds on d^2
@@ -221,6 +223,14 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
         while (I > 0 && I * T - (I * (I - 1)) / 2 > p) --I;
         while (I + 1 < T && (I + 1) * T - ((I + 1) * I) / 2 <= p) ++I;
         const long long J = I + (p - (I * T - (I * (I - 1)) / 2));
+        {
+            // lower bound of the minimum-image distance between the two tiles
+            const float4 ci = R.box[2 * I], hi = R.box[2 * I + 1], cj = R.box[2 * J], hj = R.box[2 * J + 1];
+            const float gx = fmaxf(0.f, fabsf(mic1(ci.x - cj.x, R.L0)) - (hi.x + hj.x));
+            const float gy = fmaxf(0.f, fabsf(mic1(ci.y - cj.y, R.L1)) - (hi.y + hj.y));
+            const float gz = fmaxf(0.f, fabsf(mic1(ci.z - cj.z, R.L2)) - (hi.z + hj.z));
+            if (gx * gx + gy * gy + gz * gz > R.skip2) continue;   // every pair is beyond L/2
+        }
         __syncthreads();
         const int j0 = (int)(J * R.tile);
         for (int x = threadIdx.x; x < R.tile; x += blockDim.x) {
@@ -239,16 +249,39 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
             const int *clsi = cls + ti * R.nt;
             const int jn = min(R.tile, R.N - j0);
             const int jstart = (I == J) ? (threadIdx.x + 1) : 0;
-            for (int jj = jstart; jj < jn; ++jj) {
+            // four pairs per iteration: the LDS reads and the dependent bin
+            // arithmetic of the four overlap
+            int jj = jstart;
+            for (; jj + 4 <= jn; jj += 4) {
+                float4 rj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rj[u] = tj[jj + u];
+                float d2[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float dx, dy, dz;
+                    d2[u] = dist_sq_mic(rj[u].x, rj[u].y, rj[u].z, ri.x, ri.y, ri.z, R.L0, R.L1, R.L2, dx, dy, dz);
+                }
+                int k[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) k[u] = min((int)(__builtin_amdgcn_sqrtf(d2[u]) * R.inv_bw), bmax);
+                float2 th[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) th[u] = thr[k[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k[u] - (d2[u] < th[u].x ? 1 : 0) + (d2[u] >= th[u].y ? 1 : 0);
+                    if (d2[u] < out_thr) atomicAdd(&hist[clsi[__float_as_int(rj[u].w)] + kk], 1u);
+                }
+            }
+            for (; jj < jn; ++jj) {
                 const float4 rj = tj[jj];
                 float dx, dy, dz;
                 const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, R.L0, R.L1, R.L2, dx, dy, dz);
                 if (d2 < out_thr) {
-                    int k = (int)(__builtin_amdgcn_sqrtf(d2) * R.inv_bw);
-                    k = min(k, bmax);
+                    int k = min((int)(__builtin_amdgcn_sqrtf(d2) * R.inv_bw), bmax);
                     const float2 th = thr[k];
                     k = k - (d2 < th.x ? 1 : 0) + (d2 >= th.y ? 1 : 0);
-                    k = max(0, min(k, bmax));
                     atomicAdd(&hist[clsi[__float_as_int(rj.w)] + k], 1u);
                 }
             }
@@ -257,6 +290,51 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
     __syncthreads();
     GMEM unsigned *dst = R.part + (size_t)blockIdx.x * nh;
     for (int x = threadIdx.x; x < nh; x += blockDim.x) dst[x] = hist[x];
+}
+
+// Bounding box of each tile of 256 consecutive particles, in the frame of the
+// tile's first particle (minimum-image displacements), so a tile that
+// straddles the periodic boundary still gets a tight box.
+__global__ void __launch_bounds__(256) k_rdf_boxes(const float4 *pos_p, int N, float L0, float L1, float L2,
+                                                   float4 *box_p) {
+    const GMEM float4 *pos = (const GMEM float4 *)pos_p;
+    GMEM float4 *box = (GMEM float4 *)box_p;
+    __shared__ float red[6][4];
+    const int t = blockIdx.x, i0 = t * 256, i = i0 + threadIdx.x;
+    const float4 r0 = pos[i0];
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+    if (i < N) {
+        const float4 r = pos[i];
+        const float d[3] = {mic1(r.x - r0.x, L0), mic1(r.y - r0.y, L1), mic1(r.z - r0.z, L2)};
+        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = d[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], o, 64));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o, 64));
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+            red[k][w] = lo[k];
+            red[3 + k][w] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float c[3], h[3];
+        for (int k = 0; k < 3; ++k) {
+            float a = red[k][0], b = red[3 + k][0];
+            for (int x = 1; x < 4; ++x) {
+                a = fminf(a, red[k][x]);
+                b = fmaxf(b, red[3 + k][x]);
+            }
+            h[k] = 0.5f * (b - a) * 1.0001f + 1e-4f;   // rounding margin
+            c[k] = 0.5f * (a + b);
+        }
+        box[2 * t] = make_float4(r0.x + c[0], r0.y + c[1], r0.z + c[2], 0.f);
+        box[2 * t + 1] = make_float4(h[0], h[1], h[2], 0.f);
+    }
 }
 
 // Sum the per-block histograms: thread (x, chunk) adds 'per' blocks of bin x.
@@ -274,6 +352,113 @@ __global__ void k_rdf_reduce(const unsigned *part_p, int nblocks, int nh, int pe
 __global__ void k_pack_pos(const float *r, const int *ptype, int N, float4 *pos) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) pos[i] = make_float4(r[3 * i], r[3 * i + 1], r[3 * i + 2], __int_as_float(ptype[i]));
+}
+
+// ============================================================================
+// Renumbering of the particles in cell order (gather locality for the step,
+// rebuild and histogram kernels).  Deterministic: cells in index order,
+// particles of a cell in their previous order.
+// ============================================================================
+// Morton code of a cell's coordinates (bits interleaved x, y, z): cells close
+// in space get close keys, so a run of consecutive particles is a compact
+// block of cells
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_sort_count(const DP *__restrict__ Pp, int cur, int *key, int *cnt) {
+    const DP &P = *Pp;
+    cpos_t pos = cur ? P.pos1 : P.pos0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += gridDim.x * blockDim.x) {
+        const float4 r = pos[i];
+        const int c = cell_of(P, r.x, r.y, r.z);
+        const unsigned cz = c % P.ncz, cy = (c / P.ncz) % P.ncy, cx = c / (P.ncz * P.ncy);
+        const int k = (int)((spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz));
+        key[i] = k;
+        atomicAdd(cnt + k, 1);
+    }
+}
+
+// exclusive prefix sum of a[0, n) in place (one block of 1024 threads)
+__global__ void __launch_bounds__(1024) k_scan(int *a, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = min(n, t * per), e = min(n, b + per);
+    int sum = 0;
+    for (int k = b; k < e; ++k) sum += a[k];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int k = b; k < e; ++k) {
+        const int v = a[k];
+        a[k] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sort_scatter(int N, const int *key, int *cursor, int *order) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        order[atomicAdd(cursor + key[i], 1)] = i;
+}
+
+// particles of one cell in ascending previous index
+__global__ void __launch_bounds__(256) k_sort_cells(const int *start, int ncells, int N, int *order) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+        const int b = start[c], e = (c + 1 < ncells) ? start[c + 1] : N;
+        for (int x = b + 1; x < e; ++x) {
+            const int v = order[x];
+            int y = x - 1;
+            while (y >= b && order[y] > v) {
+                order[y + 1] = order[y];
+                --y;
+            }
+            order[y + 1] = v;
+        }
+    }
+}
+
+// new[k] = old[order[k]] for every per-particle array; the positions go to
+// the other ping-pong buffer (the caller flips buf0)
+struct PermBufs {
+    float4 *vel, *frc, *rref, *obeta;
+    float *sc;
+    int *rim, *perm;
+};
+
+__global__ void __launch_bounds__(256) k_permute(const DP *__restrict__ Pp, int cur, const int *order, PermBufs T) {
+    const DP &P = *Pp;
+    const int N = P.N;
+    cpos_t src = cur ? P.pos1 : P.pos0;
+    pos_t dst = cur ? P.pos0 : P.pos1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+        const int o = order[k];
+        dst[k] = src[o];
+        T.vel[k] = P.vel[o];
+        T.frc[k] = P.frc[o];
+        T.rref[k] = P.rref[o];
+        T.obeta[k] = P.obeta[o];
+        for (int c = 0; c < 5; ++c) T.sc[(size_t)c * N + k] = P.sc[(size_t)c * N + o];
+        for (int c = 0; c < 3; ++c) T.rim[(size_t)c * N + k] = P.rim[(size_t)c * N + o];
+        T.perm[k] = P.perm[o];
+    }
+}
+
+// inverse of the permutation: order[perm[i]] = i (the caller's order back)
+__global__ void __launch_bounds__(256) k_perm_inverse(const DP *__restrict__ Pp, int *order) {
+    const DP &P = *Pp;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += gridDim.x * blockDim.x) order[P.perm[i]] = i;
 }
 
 // ============================================================================
@@ -390,6 +575,13 @@ struct gmd_ctx {
     int rebuild_cap = 0;
     float *saver_out = nullptr;
     size_t nbl_len = 0;
+    // renumbering in cell order
+    int *perm = nullptr, *skey = nullptr, *sorder = nullptr, *scnt = nullptr, *scur = nullptr;
+    PermBufs tb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int nkeys = 0;                // Morton key space of the renumbering
+    int reorder_every = 0;        // steps between renumberings (0: never)
+    long long since_reorder = 0;  // steps since the last renumbering
+    bool permuted = false;        // perm is not the identity
     // state
     int buf0 = 0;
     long long lstep = 0;
@@ -408,6 +600,7 @@ struct gmd_ctx {
 struct gmd_rdf {
     int N, nt, bins, nclass;
     float4 *pos = nullptr;
+    float4 *tbox = nullptr;   // tile boxes
     int *ptype = nullptr;
     float *d_thr = nullptr;
     unsigned *part = nullptr;
@@ -447,7 +640,9 @@ static int ctx_free(gmd_ctx *c) {
     if (c->graph) hipGraphDestroy(c->graph);
     void *ptrs[] = {c->pos0, c->pos1, c->vel, c->frc, c->rref, c->obeta, c->sc, c->rim, c->nbl, c->nbc,
                     c->ccount, c->cmem, c->d_excl,
-                    c->d_params, c->ctrl, c->partial, c->rebuild_flag, c->saver_out, c->d_P};
+                    c->d_params, c->ctrl, c->partial, c->rebuild_flag, c->saver_out, c->d_P,
+                    c->perm, c->skey, c->sorder, c->scnt, c->scur, c->tb.vel, c->tb.frc, c->tb.rref, c->tb.obeta,
+                    c->tb.sc, c->tb.rim, c->tb.perm};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -458,6 +653,8 @@ static int ctx_free(gmd_ctx *c) {
 }
 
 static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_stream);
+static int unpermute(gmd_ctx *c);
+static int host_perm(gmd_ctx *c, std::vector<int> &perm);
 
 int gmd_create(const gmd_desc *desc, gmd_ctx **out) { return ctx_create(desc, out, nullptr); }
 
@@ -595,6 +792,32 @@ static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_st
     const int max_grid = std::max(c->grid_step, c->cus * 32);
     TRY(dalloc(&c->partial, (size_t)max_grid * GMD_NRED));
     TRY(dalloc(&c->saver_out, (size_t)std::max(1, d.scalar_saves_per_block) * 6));
+    TRY(dalloc(&c->perm, N));
+    {
+        std::vector<int> iota(N);
+        for (int i = 0; i < N; ++i) iota[i] = i;
+        HTRY(hipMemcpy(c->perm, iota.data(), (size_t)N * sizeof(int), hipMemcpyHostToDevice));
+    }
+    int side = 1;
+    while (side < std::max(ncx, std::max(ncy, ncz))) side *= 2;
+    if (d.nblist_kind == GMD_NBLIST_LINKED_LISTS && c->excl.empty() && side <= 256) {
+        // renumbering buffers (the exclusion table is indexed by particle, so
+        // systems with exclusions keep the caller's numbering)
+        TRY(dalloc(&c->skey, N));
+        TRY(dalloc(&c->sorder, N));
+        c->nkeys = side * side * side;
+        TRY(dalloc(&c->scnt, (size_t)c->nkeys));
+        TRY(dalloc(&c->scur, (size_t)c->nkeys));
+        TRY(dalloc(&c->tb.vel, N));
+        TRY(dalloc(&c->tb.frc, N));
+        TRY(dalloc(&c->tb.rref, N));
+        TRY(dalloc(&c->tb.obeta, N));
+        TRY(dalloc(&c->tb.sc, (size_t)5 * N));
+        TRY(dalloc(&c->tb.rim, (size_t)3 * N));
+        TRY(dalloc(&c->tb.perm, N));
+        c->reorder_every = 400;
+        c->since_reorder = 1LL << 40;
+    }
     if (!c->excl.empty()) {
         TRY(dalloc(&c->d_excl, c->excl.size()));
         HTRY(hipMemcpy(c->d_excl, c->excl.data(), c->excl.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -678,11 +901,13 @@ static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_st
     P.saves_per_block = d.scalar_saves_per_block;
     P.ctrl = (GMEM Ctrl *)c->ctrl;
     P.partial = (GMEM float *)c->partial;
+    P.pstride = max_grid;
     P.saver_out = (GMEM float *)c->saver_out;
     P.lds_pp = (int)lds_pp;
     P.lds_red = (int)lds_red;
     P.lds_build = (int)lds_b;
     P.step_grid = c->grid_step;
+    P.perm = (const GMEM int *)c->perm;
 
     TRY(dalloc(&c->d_P, 1));
     c->kern = pick_kernels(d.pair_base, d.cutoff, d.integrator);
@@ -711,6 +936,7 @@ int gmd_destroy(gmd_ctx *ctx) {
 int gmd_upload(gmd_ctx *c, const float *r, const float *v, const float *f, const float *scalars, const int *ptype,
                const int *r_im) {
     if (!c) return fail("gmd_upload: null context");
+    if (unpermute(c)) return -1;   // host arrays come in the caller's numbering
     const int N = c->d.N;
     const gmd_desc &d = c->d;
     if (ptype)
@@ -767,34 +993,40 @@ int gmd_download(gmd_ctx *c, float *r, float *v, float *f, float *scalars, int *
     const int N = c->d.N;
     const gmd_desc &d = c->d;
     HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<int> perm;   // internal index -> caller's index (empty: identity)
+    if (host_perm(c, perm)) return -1;
+    auto at = [&](int i) { return perm.empty() ? i : perm[i]; };
     if (r || ptype) {
         HIPCHECK(hipMemcpy(c->h_pos.data(), c->buf0 ? c->pos1 : c->pos0, (size_t)N * sizeof(float4),
                            hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i) {
+            const int o = at(i);
             if (r) {
-                r[3 * i] = c->h_pos[i].x;
-                r[3 * i + 1] = c->h_pos[i].y;
-                r[3 * i + 2] = c->h_pos[i].z;
+                r[3 * o] = c->h_pos[i].x;
+                r[3 * o + 1] = c->h_pos[i].y;
+                r[3 * o + 2] = c->h_pos[i].z;
             }
-            if (ptype) ptype[i] = f2i(c->h_pos[i].w);
+            if (ptype) ptype[o] = f2i(c->h_pos[i].w);
         }
     }
     if (v || scalars) {
         HIPCHECK(hipMemcpy(c->h_vel.data(), c->vel, (size_t)N * sizeof(float4), hipMemcpyDeviceToHost));
         if (v)
             for (int i = 0; i < N; ++i) {
-                v[3 * i] = c->h_vel[i].x;
-                v[3 * i + 1] = c->h_vel[i].y;
-                v[3 * i + 2] = c->h_vel[i].z;
+                const int o = at(i);
+                v[3 * o] = c->h_vel[i].x;
+                v[3 * o + 1] = c->h_vel[i].y;
+                v[3 * o + 2] = c->h_vel[i].z;
             }
     }
     if (f) {
         std::vector<float4> h(N);
         HIPCHECK(hipMemcpy(h.data(), c->frc, (size_t)N * sizeof(float4), hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i) {
-            f[3 * i] = h[i].x;
-            f[3 * i + 1] = h[i].y;
-            f[3 * i + 2] = h[i].z;
+            const int o = at(i);
+            f[3 * o] = h[i].x;
+            f[3 * o + 1] = h[i].y;
+            f[3 * o + 2] = h[i].z;
         }
     }
     if (scalars) {
@@ -803,14 +1035,14 @@ int gmd_download(gmd_ctx *c, float *r, float *v, float *f, float *scalars, int *
         const int sids[5] = {d.sid_u, d.sid_w, d.sid_lap, d.sid_k, d.sid_fsq};
         for (int col = 0; col < 5; ++col)
             if (sids[col] >= 0)
-                for (int i = 0; i < N; ++i) scalars[(size_t)i * d.nscalars + sids[col]] = h[(size_t)col * N + i];
+                for (int i = 0; i < N; ++i) scalars[(size_t)at(i) * d.nscalars + sids[col]] = h[(size_t)col * N + i];
         if (d.sid_m >= 0)
-            for (int i = 0; i < N; ++i) scalars[(size_t)i * d.nscalars + d.sid_m] = c->h_vel[i].w;
+            for (int i = 0; i < N; ++i) scalars[(size_t)at(i) * d.nscalars + d.sid_m] = c->h_vel[i].w;
     }
     if (r_im) {
         HIPCHECK(hipMemcpy(c->h_rim.data(), c->rim, c->h_rim.size() * sizeof(int), hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i)
-            for (int k = 0; k < 3; ++k) r_im[3 * i + k] = c->h_rim[(size_t)k * N + i];
+            for (int k = 0; k < 3; ++k) r_im[3 * at(i) + k] = c->h_rim[(size_t)k * N + i];
     }
     return 0;
 }
@@ -836,11 +1068,74 @@ static int ensure_flags(gmd_ctx *c, int steps) {
     return 0;
 }
 
+// Renumber the particles by applying 'order' (new k = old order[k]) to every
+// per-particle array; the positions move to the other ping-pong buffer.
+static int apply_order(gmd_ctx *c, const int *order) {
+    hipStream_t st = c->stream;
+    const int N = c->d.N;
+    const int grid = std::max(1, std::min((N + 255) / 256, c->cus * 8));
+    hipLaunchKernelGGL(k_permute, dim3(grid), dim3(256), 0, st, (const DP *)c->d_P, c->buf0, order, c->tb);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(c->vel, c->tb.vel, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(c->frc, c->tb.frc, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(c->rref, c->tb.rref, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(c->obeta, c->tb.obeta, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(c->sc, c->tb.sc, (size_t)5 * N * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(c->rim, c->tb.rim, (size_t)3 * N * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(c->perm, c->tb.perm, (size_t)N * sizeof(int), hipMemcpyDeviceToDevice, st));
+    c->buf0 ^= 1;
+    c->need_rebuild = true;   // list indices refer to the old numbering
+    return 0;
+}
+
+// cell order of the current positions
+static int reorder(gmd_ctx *c) {
+    hipStream_t st = c->stream;
+    const int N = c->d.N, nc = c->nkeys;
+    const int grid = std::max(1, std::min((N + 255) / 256, c->cus * 8));
+    const int gridc = std::max(1, std::min((nc + 255) / 256, c->cus * 8));
+    HIPCHECK(hipMemsetAsync(c->scnt, 0, (size_t)nc * sizeof(int), st));
+    hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(256), 0, st, (const DP *)c->d_P, c->buf0, c->skey, c->scnt);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->scnt, nc);
+    HIPCHECK(hipMemcpyAsync(c->scur, c->scnt, (size_t)nc * sizeof(int), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, st, N, (const int *)c->skey, c->scur, c->sorder);
+    hipLaunchKernelGGL(k_sort_cells, dim3(gridc), dim3(256), 0, st, (const int *)c->scnt, nc, N, c->sorder);
+    HIPCHECK(hipGetLastError());
+    if (apply_order(c, c->sorder)) return -1;
+    c->permuted = true;
+    c->since_reorder = 0;
+    return 0;
+}
+
+// back to the caller's numbering (before host data arrive in that order)
+static int unpermute(gmd_ctx *c) {
+    if (!c->permuted) return 0;
+    const int N = c->d.N;
+    const int grid = std::max(1, std::min((N + 255) / 256, c->cus * 8));
+    hipLaunchKernelGGL(k_perm_inverse, dim3(grid), dim3(256), 0, c->stream, (const DP *)c->d_P, c->sorder);
+    HIPCHECK(hipGetLastError());
+    if (apply_order(c, c->sorder)) return -1;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->permuted = false;
+    return 0;
+}
+
+// the permutation on the host (internal index -> caller's index), or empty
+static int host_perm(gmd_ctx *c, std::vector<int> &perm) {
+    perm.clear();
+    if (!c->permuted) return 0;
+    perm.resize(c->d.N);
+    HIPCHECK(hipMemcpy(perm.data(), c->perm, (size_t)c->d.N * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 // per-run set-up of one system on its stream: control block, rebuild flags,
 // saver rows, the cell grid of the starting positions and the skin test
 static int run_prepare(gmd_ctx *c, float t0, int steps) {
     if (steps < 0) return fail("gmd_run: steps must be non-negative");
     if (ensure_flags(c, steps)) return -1;
+    if (c->reorder_every > 0 && steps > 0 && c->since_reorder >= c->reorder_every)
+        if (reorder(c)) return -1;
     hipStream_t st = c->stream;
     const DP *dP = c->d_P;
     HIPCHECK(hipMemsetAsync(c->rebuild_flag, 0, (size_t)(steps + 2) * sizeof(int), st));
@@ -864,6 +1159,7 @@ static int run_prepare(gmd_ctx *c, float t0, int steps) {
 static int run_complete(gmd_ctx *c, int steps) {
     c->buf0 ^= (steps & 1);
     c->lstep += steps;
+    c->since_reorder += steps;
     c->need_rebuild = false;
     Ctrl h;
     HIPCHECK(hipMemcpy(&h, c->ctrl, sizeof h, hipMemcpyDeviceToHost));
@@ -987,13 +1283,25 @@ int gmd_read_nblist(gmd_ctx *c, int *out) {
     std::vector<int> nbl(c->nbl_len), nbc(N);
     HIPCHECK(hipMemcpy(nbl.data(), c->nbl, nbl.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(nbc.data(), c->nbc, (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> perm;
+    if (host_perm(c, perm)) return -1;
+    auto at = [&](int i) { return perm.empty() ? i : perm[i]; };
     for (int i = 0; i < N; ++i) {
-        int *row = out + (size_t)i * (M + 1);
+        int *row = out + (size_t)at(i) * (M + 1);
         int n = std::min(nbc[i], M);
-        for (int k = 0; k < n; ++k) row[k] = nbl[((size_t)(k / tp) * N + i) * tp + (k % tp)];
+        for (int k = 0; k < n; ++k) row[k] = at(nbl[((size_t)(k / tp) * N + i) * tp + (k % tp)]);
         for (int k = n; k < M; ++k) row[k] = 0;
         row[M] = nbc[i];
     }
+    return 0;
+}
+
+int gmd_set_reorder_interval(gmd_ctx *c, int steps) {
+    if (!c) return fail("gmd_set_reorder_interval: null context");
+    if (steps < 0) return fail("gmd_set_reorder_interval: steps must be non-negative");
+    if (steps > 0 && !c->sorder)
+        return fail("gmd_set_reorder_interval: renumbering needs linked lists and no exclusions");
+    c->reorder_every = steps;
     return 0;
 }
 
@@ -1086,8 +1394,9 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const long long T = (N + 255) / 256;
     const long long npairs = T * (T + 1) / 2;
-    r->grid = (int)std::max(1LL, std::min(npairs, (long long)cus * 2));
+    r->grid = (int)std::max(1LL, std::min(npairs, (long long)cus * 4));
     if (hipMalloc(&r->pos, (size_t)N * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&r->tbox, (size_t)2 * T * sizeof(float4)) != hipSuccess ||
         hipMalloc(&r->ptype, (size_t)N * sizeof(int)) != hipSuccess ||
         hipMalloc(&r->d_thr, (size_t)bins * sizeof(float2)) != hipSuccess ||
         hipMalloc(&r->part, (size_t)r->grid * nh * sizeof(unsigned)) != hipSuccess ||
@@ -1105,7 +1414,7 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
 
 int gmd_rdf_destroy(gmd_rdf *r) {
     if (!r) return 0;
-    void *ptrs[] = {r->pos, r->ptype, r->d_thr, r->part, r->d_out};
+    void *ptrs[] = {r->pos, r->tbox, r->ptype, r->d_thr, r->part, r->d_out};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (r->stream) hipStreamDestroy(r->stream);
@@ -1142,8 +1451,12 @@ static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const floa
     R.thr2 = (const GMEM float2 *)r->d_thr;
     R.out_thr = r->thr[r->bins];
     R.part = (GMEM unsigned *)r->part;
+    R.box = (const GMEM float4 *)r->tbox;
+    R.skip2 = R.out_thr;
     const size_t nh = (size_t)r->nclass * r->bins;
     HIPCHECK(hipMemsetAsync(r->d_out, 0, nh * sizeof(unsigned long long), r->stream));
+    hipLaunchKernelGGL(k_rdf_boxes, dim3(R.ntiles), dim3(256), 0, r->stream, pos, r->N, box[0], box[1], box[2],
+                       r->tbox);
     hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), rdf_lds_bytes(r->nclass, r->bins, r->nt), r->stream, R);
     HIPCHECK(hipGetLastError());
     const int per = 16;
@@ -1174,7 +1487,11 @@ int gmd_rdf_compute_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, unsigned long 
     if (ptype) {
         for (int i = 0; i < r->N; ++i)
             if (ptype[i] < 0 || ptype[i] >= r->nt) return fail("gmd_rdf_compute_ctx: type out of range");
-        HIPCHECK(hipMemcpy(r->ptype, ptype, (size_t)r->N * sizeof(int), hipMemcpyHostToDevice));
+        std::vector<int> perm, pt(ptype, ptype + r->N);
+        if (host_perm(c, perm)) return -1;
+        if (!perm.empty())
+            for (int i = 0; i < r->N; ++i) pt[i] = ptype[perm[i]];   // types in the context's numbering
+        HIPCHECK(hipMemcpy(r->ptype, pt.data(), (size_t)r->N * sizeof(int), hipMemcpyHostToDevice));
         dp = r->ptype;
     } else if (c->d.ntypes > r->nt) {
         return fail("gmd_rdf_compute_ctx: context has %d types, calculator %d", c->d.ntypes, r->nt);

@@ -9,6 +9,10 @@
 //   rref    float4 positions at the last neighbour-list rebuild
 //   nbl     int    neighbour list, entry k of particle i at ((k/tp)*N + i)*tp + k%tp
 //   nbc     int    neighbour count (true count; entries beyond max_nbs are dropped)
+//   perm    int    internal index -> caller's index; the particles are renumbered
+//                  in cell order at run starts (gather locality), every array
+//                  above is in internal order and the C ABI converts on upload /
+//                  download
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -92,11 +96,13 @@ struct DP {
     int compute_u, compute_w, compute_lap, compute_k, compute_fsq;
     int mom_every, save_every, saves_per_block;
     GMEM Ctrl *ctrl;
-    GMEM float *partial;       // [nblocks][GMD_NRED]
+    GMEM float *partial;       // [GMD_NRED][pstride]: sum c of block b at c * pstride + b
+    int pstride;
     GMEM int *rebuild_flag;    // [nsteps+2]
     GMEM float *saver_out;     // [saves_per_block][6]
     int lds_pp, lds_red, lds_build;   // byte offsets into dynamic LDS
     int step_grid;             // blocks of the step kernel (partial sums to reduce)
+    const GMEM int *perm;      // internal index -> index in the caller's order (particles are kept in cell order)
 };
 
 // ---------------------------------------------------------------------------

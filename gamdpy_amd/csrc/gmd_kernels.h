@@ -122,6 +122,8 @@ struct StepInfo {
     float sh0, sh1, sh2;
     float xi;
     float lang_amp;
+    float t0;
+    int nsteps;
     long long lstep;
 };
 
@@ -134,7 +136,9 @@ __device__ __forceinline__ StepInfo step_info(const DP &P, int s, int integrator
     I.cbuf_next = (s + 1) % 3;
     I.cbuf_clear = (s + 2) % 3;
     I.rebuild = rebuild;
-    I.integrate = (s < C->nsteps) && integrator != 0;
+    I.nsteps = C->nsteps;
+    I.t0 = C->t0;
+    I.integrate = (s < I.nsteps) && integrator != 0;
     I.need_ksum = I.integrate && integrator == 2;
     I.need_mom = I.integrate && P.mom_every > 0 && (s % P.mom_every) == 0;
     I.need_save = I.integrate && P.save_every > 0 && (s % P.save_every) == 0 && (s / P.save_every) < P.saves_per_block;
@@ -146,7 +150,7 @@ __device__ __forceinline__ StepInfo step_info(const DP &P, int s, int integrator
     I.lstep = C->lstep0 + s;
     I.lang_amp = 0.f;
     if (integrator == 3) {
-        double t = (double)C->t0 + (double)s * (double)P.dt;
+        double t = (double)I.t0 + (double)s * (double)P.dt;
         float T = (float)target_temperature(P, t);
         {
 #pragma clang fp contract(off)
@@ -224,7 +228,7 @@ __device__ __forceinline__ void particle_finish(const DP &P, const StepInfo &I, 
             vz = plus * (minus * vz + (fz / m) * dt); K = K + ((0.5f * m) * vz) * vz; rz = rz + vz * dt;
         } else {  // NVT_Langevin.py:363-378
             float z0, z1, z2;
-            normal3(P.seed, I.lstep, i, z0, z1, z2);
+            normal3(P.seed, I.lstep, P.perm[i], z0, z1, z2);   // keyed by the caller's particle index
             float4 ob = P.obeta[i];
             const float num = 2.0f * m - P.alpha * dt;
             const float den = 2.0f * m + P.alpha * dt;
@@ -555,22 +559,26 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
 }
 
 // ---------------------------------------------------------------------------
-// Rebuild step, N squared (nblist.py:89-143): positions staged through LDS
-// tiles, one thread per particle, neighbours appended in ascending j; pairs
-// inside the cutoff contribute to the force at once, then the particle is
-// integrated.  Every thread of every block runs the same number of rounds.
+// Rebuild step, N squared (nblist.py:89-143): one wavefront per particle i.
+// The block stages 256 positions at a time in LDS; each wavefront tests them
+// against its particle 64 at a time and appends the accepted j with a ballot,
+// so the list is in ascending j; pairs inside the cutoff contribute to the
+// force at once, and lane 0 integrates the particle.  Every wavefront of
+// every block runs the same number of rounds (the tiles are shared).
 // ---------------------------------------------------------------------------
 template <int BASE, int CUT, int INTEG>
-__device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp, float4 *tile,
-                                        float acc[GMD_NRED]) {
+__device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp,
+                                                 float4 *tile, float acc[GMD_NRED]) {
     const int bd = blockDim.x;
-    const int per_round = B.nb * bd;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, wpb = bd >> 6;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int per_round = B.nb * wpb;
     const int rounds = (P.N + per_round - 1) / per_round;
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
     const bool lap = P.compute_lap != 0;
     for (int rd = 0; rd < rounds; ++rd) {
-        const int i = (rd * B.nb + B.bid) * bd + threadIdx.x;
+        const int i = (rd * B.nb + B.bid) * wpb + wid;   // wave-uniform
         const bool valid = i < P.N;
         float4 ri = make_float4(0.f, 0.f, 0.f, 0.f), vi = make_float4(0.f, 0.f, 0.f, 1.f);
         if (valid) {
@@ -582,31 +590,44 @@ __device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B,
         float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
         for (int t0 = 0; t0 < P.N; t0 += bd) {
             __syncthreads();
-            const int jl = t0 + threadIdx.x;
+            const int jl = t0 + (int)threadIdx.x;
             if (jl < P.N) tile[threadIdx.x] = pos[jl];
             __syncthreads();
             const int tn = min(bd, P.N - t0);
-            if (valid) {
-                for (int jj = 0; jj < tn; ++jj) {
-                    const int j = t0 + jj;
-                    if (!pair_accept(P, i, j)) continue;
-                    const float4 rj = tile[jj];
+            if (!valid) continue;
+            for (int c0 = 0; c0 < tn; c0 += 64) {
+                const int j = t0 + c0 + lane;
+                bool ok = false;
+                if (c0 + lane < tn && pair_accept(P, i, j)) {
+                    const float4 rj = tile[c0 + lane];
                     float dx, dy, dz;
                     const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
-                    if (d2 < P.cps2 && !excluded(P, i, j)) {
-                        if (cnt < P.max_nbs) P.nbl[nbl_index(P, i, cnt)] = j;
-                        ++cnt;
-                        const PairP &q = ppi[__float_as_int(rj.w)];
-                        if (d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
-                    }
+                    ok = d2 < P.cps2 && !excluded(P, i, j);
+                    const PairP &q = ppi[__float_as_int(rj.w)];
+                    if (ok && d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
                 }
+                const unsigned long long mask = __ballot(ok);
+                if (ok) {
+                    const int slot = cnt + __popcll(mask & lt_mask);
+                    if (slot < P.max_nbs) P.nbl[nbl_index(P, i, slot)] = j;
+                }
+                cnt += __popcll(mask);
             }
         }
         if (valid) {
-            P.nbc[i] = cnt;
-            P.rref[i] = ri;
-            if (cnt > P.max_nbs) atomicMax(gen_ptr(&P.ctrl->nbflag[1]), cnt);
-            particle_finish<INTEG>(P, I, i, ri, vi, ri, fx, fy, fz, uu * 0.5f, ww * (0.5f / 3.0f), ll, true, acc);
+            fx = wave_sum_f(fx);
+            fy = wave_sum_f(fy);
+            fz = wave_sum_f(fz);
+            uu = wave_sum_f(uu);
+            ww = wave_sum_f(ww);
+            if (lap) ll = wave_sum_f(ll);
+            if (lane == 0) {
+                P.nbc[i] = cnt;
+                P.rref[i] = ri;
+                if (cnt > P.max_nbs) atomicMax(gen_ptr(&P.ctrl->nbflag[1]), cnt);
+            }
+            particle_finish<INTEG>(P, I, i, ri, vi, ri, fx, fy, fz, uu * 0.5f, ww * (0.5f / 3.0f), ll, lane == 0,
+                                   acc);
         }
     }
 }
@@ -642,7 +663,7 @@ __device__ __forceinline__ void block_partials(const DP &P, const BlockCtx &B, f
     if (threadIdx.x < GMD_NRED) {
         double sum = 0.0;
         for (int w = 0; w < nw; ++w) sum += red[w * GMD_NRED + threadIdx.x];
-        P.partial[(size_t)B.bid * GMD_NRED + threadIdx.x] = (float)sum;
+        P.partial[(size_t)threadIdx.x * P.pstride + B.bid] = (float)sum;
     }
     __syncthreads();
 }
@@ -654,13 +675,18 @@ __device__ __forceinline__ void block_partials(const DP &P, const BlockCtx &B, f
 __device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, const int nblocks) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const bool need = I.need_ksum || I.need_mom || I.need_save;
+    GMEM Ctrl *C = P.ctrl;
+    // the rebuild flag thread 0 publishes at the end is loaded first, so the
+    // load overlaps with the reduction (rebuild_flag holds nsteps+2 entries)
+    int next_flag = 0;
+    if (threadIdx.x == 0) next_flag = ld_flag(P.rebuild_flag + I.s + 1);
     if (need) {
         double acc[GMD_NRED];
 #pragma unroll
         for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
-            for (int c = 0; c < GMD_NRED; ++c) acc[c] += (double)P.partial[(size_t)b * GMD_NRED + c];
+            for (int c = 0; c < GMD_NRED; ++c) acc[c] += (double)P.partial[(size_t)c * P.pstride + b];
         }
 #pragma unroll
         for (int c = 0; c < GMD_NRED; ++c) {
@@ -676,14 +702,13 @@ __device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, co
             if (need)
                 for (int w = 0; w < nw; ++w) tot[c] += red[w * GMD_NRED + c];
         }
-        GMEM Ctrl *C = P.ctrl;
         if (I.need_ksum) {
             const float ksum = (float)tot[RED_K];
-            const double t = (double)C->t0 + (double)I.s * (double)P.dt;
+            const double t = (double)I.t0 + (double)I.s * (double)P.dt;
             const double T = target_temperature(P, t);
             const double dev = (double)(2.0f * ksum) / (P.degrees * T) - 1.0;
             const double inc = (double)(P.dt * P.omega2) * dev;
-            C->xi = (float)((double)C->xi + inc);
+            C->xi = (float)((double)I.xi + inc);
             C->ksum = 0.0f;
         }
         if (I.need_mom) {
@@ -703,7 +728,7 @@ __device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, co
             row[5] = P.box0 * P.box1 * P.box2;
         }
         if (I.rebuild) C->nbflag[2] += 1;
-        C->rebuild_now = (I.s < C->nsteps) ? ld_flag(P.rebuild_flag + I.s + 1) : 0;
+        C->rebuild_now = (I.s < I.nsteps) ? next_flag : 0;
         C->step = I.s + 1;
     }
     __syncthreads();

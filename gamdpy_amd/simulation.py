@@ -33,8 +33,12 @@ def get_default_compute_plan(configuration):
       skin neighbour-list skin,   UtilizeNIII  (half lists; not used on MI355X),
       gridsync  one persistent cooperative launch per timeblock,
       nblist    'N squared' or 'linked lists'.
-    The lane count fills the chip with at least 16 wavefronts per CU; the
-    persistent launch is used while the grid stays co-resident.
+    The lane count fills the chip with at least 16 wavefronts per CU.  The
+    default driver is one launch per step replayed from a hipGraph: on MI355X
+    a graph kernel launch costs about 1.6 us while a software grid barrier
+    costs 10 us (256 blocks) to 40-70 us (1024 blocks), so the graph is as
+    fast or faster at every size measured (N = 2048 to 65536); gridsync=True
+    selects the persistent cooperative kernel.
     """
     N = configuration.N
     try:
@@ -46,7 +50,7 @@ def get_default_compute_plan(configuration):
         tp *= 2
     pb = max(1, 256 // tp)
     skin = 0.5
-    gridsync = N * tp <= cus * 64 * 32
+    gridsync = False
     nblist = 'N squared' if N <= 6000 else 'linked lists'
     return {'pb': pb, 'tp': tp, 'skin': skin, 'UtilizeNIII': False, 'gridsync': gridsync, 'nblist': nblist}
 

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define GMD_ABI_VERSION 4
+#define GMD_ABI_VERSION 5
 
 typedef struct gmd_ctx gmd_ctx;
 typedef struct gmd_rdf gmd_rdf;
@@ -137,6 +137,11 @@ int gmd_read_nbflag(gmd_ctx *ctx, int nbflag[3]);
 int gmd_read_nblist(gmd_ctx *ctx, int *out);
 /* force a rebuild before the next interaction evaluation (nblist.py:67-69 force_update) */
 int gmd_force_rebuild(gmd_ctx *ctx);
+/* Renumber the particles in cell order at the start of a run once 'steps'
+ * steps have passed since the last renumbering (0: never).  Internal only:
+ * every host array keeps the caller's order.  Linked-list contexts without
+ * exclusions start at 400; no counterpart in the reference. */
+int gmd_set_reorder_interval(gmd_ctx *ctx, int steps);
 
 /* NVT.thermostat_state (NVT.py:143): [xi, sum K] */
 int gmd_get_thermostat(gmd_ctx *ctx, float state[2]);
