@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-replica and adjoint measurements")
     ap.add_argument("--replicas", type=int, default=128, help="replicas of the batched measurement")
     ap.add_argument("--replica-steps", type=int, default=1000, help="steps of the batched measurement")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain copy kernel that reads MIB MiB and writes MIB MiB (untimed), "
+                         "the known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -90,6 +93,9 @@ def main():
     from gamdpy_amd import _native
     info = _native.device_info()
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['compute_units']}")
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     conf, pp = make_system()
     N = conf.N
@@ -156,8 +162,8 @@ def main():
     # library's stream, over the next PROFILE_STEPS steps of the same run
     prof = None
     if not plan['gridsync']:
-        k_ms, t_ms = ctx.profile_steps(np.float32(done * DT), PROFILE_STEPS)
-        prof = {"k_step_us": k_ms * 1e3, "k_tail_us": t_ms * 1e3, "steps": PROFILE_STEPS}
+        k_ms, _ = ctx.profile_steps(np.float32(done * DT), PROFILE_STEPS)
+        prof = {"k_step_us": k_ms * 1e3, "steps": PROFILE_STEPS}
     conf.copy_to_host()
     mean_nb, max_nb = list_length_stats(conf, pp)
 
@@ -206,6 +212,14 @@ def main():
     }
     out.update(extras)
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """Four launches of the library's plain copy kernel (k_copy) over mib MiB:
+    each reads mib MiB and writes mib MiB, nothing else."""
+    from gamdpy_amd import _native
+    ms = _native.copy_probe(mib << 20, reps=3)
+    log(f"[bench] copy calibration: k_copy of {mib} MiB, {2 * (mib << 20) / (ms * 1e-3) / 1e9:.0f} GB/s")
 
 
 def _ka_replica(gp, N, rho, T, seed):

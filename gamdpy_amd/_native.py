@@ -97,6 +97,7 @@ _SIGNATURES = {
     "gmd_read_nblist": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "gmd_force_rebuild": (ctypes.c_int, [ctypes.c_void_p]),
     "gmd_set_reorder_interval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gmd_copy_probe": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, c_float_p]),
     "gmd_get_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "gmd_set_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "gmd_read_scalar_output": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
@@ -188,6 +189,14 @@ def device_info():
                         "max_threads_per_block": info.max_threads_per_block, "total_mem": info.total_mem,
                         "name": info.name.decode(), "arch": info.arch.decode()}
     return _device_info
+
+
+def copy_probe(nbytes, reps=3):
+    """Mean milliseconds of a plain device copy of nbytes (read nbytes, write
+    nbytes) over reps launches."""
+    ms = ctypes.c_float()
+    check(load().gmd_copy_probe(int(nbytes), int(reps), ctypes.byref(ms)), "gmd_copy_probe")
+    return float(ms.value)
 
 
 def fill_desc(d, desc_fields, params, exclusions=None):

@@ -56,11 +56,13 @@ class Replica:
 
 
 def replica_plan(replica, tp, skin):
-    """Per-replica compute plan: linked lists where the box admits them and N
-    is large enough to gain from them, else N squared."""
+    """Per-replica compute plan: linked lists wherever the box admits them
+    (three cells per side), else N squared.  In a batch a replica owns a few
+    blocks, so an N-squared rebuild (N^2 / blocks pair tests) of one hot
+    replica would hold up every step of the launch; the cell walk is O(N)."""
     conf = replica.configuration
     rc = float(replica.interaction.param_table()[1])
-    ll = conf.N > 6000 and _ll_cells_ok(conf, rc, skin)
+    ll = _ll_cells_ok(conf, rc, skin)
     return {'pb': max(1, 256 // tp), 'tp': tp, 'skin': skin, 'UtilizeNIII': False, 'gridsync': False,
             'nblist': 'linked lists' if ll else 'N squared', 'graph': False}
 

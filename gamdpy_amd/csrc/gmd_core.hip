@@ -45,30 +45,40 @@ __device__ __forceinline__ void persistent_body(const DP *Pp, const BlockCtx &B)
 #pragma unroll
         for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
         step_body<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
-        if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, B, acc, red);
+        const unsigned mask = red_mask(I.need_ksum, I.need_mom, I.need_save);
+        if (mask) block_partials(P, B, acc, red, P.partial, mask);
         grid_barrier(P.ctrl, B.nb, [&]() { step_tail(P, I, red, B.nb); });
     }
 }
 
-// One step; the step index and the rebuild decision come from the control
-// block that the previous step's tail wrote.
-template <int BASE, int CUT, int INTEG>
-__device__ __forceinline__ void step_once(const DP &P, const BlockCtx &B) {
+// One step.  FOLD (one system): the step is s_base + k, the rebuild decision
+// is the flag the previous step raised, and the previous step's tail runs in
+// every block here (folded_tail); the partial sums alternate between two
+// buffers.  Otherwise (replicas of a batch) the step index and the rebuild
+// decision come from the control block that the previous step's k_batch_tail
+// wrote.
+template <int BASE, int CUT, int INTEG, bool FOLD>
+__device__ __forceinline__ void step_once(const DP &P, const BlockCtx &B, int k) {
     extern __shared__ __align__(16) char lds[];
-    const int s = P.ctrl->step;
+    const int s = FOLD ? P.ctrl->s_base + k : P.ctrl->step;
     if (s > P.ctrl->nsteps) return;
-    const bool rebuild = P.ctrl->rebuild_now != 0;
+    const bool rebuild = FOLD ? ld_flag(P.rebuild_flag + s) != 0 : P.ctrl->rebuild_now != 0;
     PairP *pp = (PairP *)(lds + P.lds_pp);
     double *red = (double *)(lds + P.lds_red);
     char *build = lds + P.lds_build;
+    StepInfo I = step_info(P, s, INTEG, rebuild);
+    float pv[RED_PRE][GMD_NRED];
+    const unsigned qmask = FOLD ? folded_tail_load<INTEG>(P, I, pv) : 0u;
     load_pair_table(P, pp);
+    if (FOLD) folded_tail<INTEG>(P, B, I, red, qmask, pv);
     __syncthreads();
-    const StepInfo I = step_info(P, s, INTEG, rebuild);
     float acc[GMD_NRED];
 #pragma unroll
     for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
     step_body<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
-    if (I.need_ksum || I.need_mom || I.need_save) block_partials(P, B, acc, red);
+    const unsigned mask = red_mask(I.need_ksum, I.need_mom, I.need_save);
+    if (mask) block_partials(P, B, acc, red, FOLD ? P.partial + (size_t)(s & 1) * GMD_NRED * P.pstride : P.partial, mask);
+    if (FOLD && rebuild && B.bid == 0 && threadIdx.x == 0) P.ctrl->nbflag[2] += 1;
 }
 
 template <int BASE, int CUT, int INTEG>
@@ -76,14 +86,21 @@ __global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
     persistent_body<BASE, CUT, INTEG>(Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
 }
 
-// One step per launch.  The cross-block work (sums, thermostat, momentum
-// shift, saver row, rebuild decision) is the separate one-block k_tail: a
-// last-block-done scheme here would need an agent-scope release per block,
-// each a write-back of the XCD's L2 under the blocks still running, which
-// costs more than one extra launch.
+// One step per launch (step s_base + k).  The cross-block work of a step
+// (sums, thermostat, momentum shift, saver row) runs at the start of the
+// next step's launch in every block (folded_tail): no one-block tail launch
+// between steps, and no last-block-done scheme, which would need an
+// agent-scope release per block (a write-back of the XCD's L2 under the
+// blocks still running).
 template <int BASE, int CUT, int INTEG>
-__global__ void __launch_bounds__(256, 4) k_step(const DP *__restrict__ Pp) {
-    step_once<BASE, CUT, INTEG>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
+__global__ void __launch_bounds__(256, 4) k_step(const DP *__restrict__ Pp, int k) {
+    step_once<BASE, CUT, INTEG, true>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x}, k);
+}
+
+// end of a captured group of steps: the next group starts 'n' steps later
+__global__ void k_advance(Ctrl *Cp, int n) {
+    GMEM Ctrl *C = (GMEM Ctrl *)Cp;
+    if (threadIdx.x == 0 && blockIdx.x == 0) C->s_base += n;
 }
 
 // Batched replicas: block b of the grid works for replica blk[b].rep as its
@@ -108,18 +125,7 @@ __global__ void __launch_bounds__(256) k_batch_step(const DP *const *Ps, const B
     const int rep = __builtin_amdgcn_readfirstlane(a.rep);
     if (rep < 0) return;
     const BlockCtx B = {__builtin_amdgcn_readfirstlane(a.bid), __builtin_amdgcn_readfirstlane(a.nb)};
-    step_once<BASE, CUT, INTEG>(*uniform_ptr(Ps[rep]), B);
-}
-
-// The cross-block tail of a step as its own one-block kernel (the kernel
-// boundary orders it after every block of k_step).
-__global__ void __launch_bounds__(256) k_tail(const DP *__restrict__ Pp, int integrator) {
-    const DP &P = *Pp;
-    __shared__ double red[4 * GMD_NRED];
-    const int s = P.ctrl->step;
-    if (s > P.ctrl->nsteps) return;
-    const StepInfo I = step_info(P, s, integrator, P.ctrl->rebuild_now != 0);
-    step_tail(P, I, red, P.step_grid);
+    step_once<BASE, CUT, INTEG, false>(*uniform_ptr(Ps[rep]), B, 0);
 }
 
 // the tails of all replicas of a batch: block r finishes replica r's step
@@ -143,6 +149,7 @@ __global__ void k_begin_run(Ctrl *Cp, int nsteps, float t0, int buf0, long long 
         C->buf0 = buf0;
         C->lstep0 = lstep0;
         C->step = 0;
+        C->s_base = 0;
         C->red_count = 0;
         C->bar_count = 0;
         C->cell_overflow = 0;
@@ -177,43 +184,113 @@ __global__ void k_setup_finish(const DP *__restrict__ Pp) {
 // ============================================================================
 // Radial distribution (calculator_radial_distribution.py:76-109)
 // ============================================================================
+#define RDF_TILE 256     // particles per tile (one block of threads; one staged LDS tile)
+#define RDF_CHUNK 64     // particles per chunk (one wavefront)
+
 struct RdfP {
-    int N, nt, bins, nclass, tile, ntiles;
-    long long npairs;
+    int N, nt, bins, nchunks, ntiles;
+    long long npairs;          // tile pairs I <= J
     float L0, L1, L2, inv_bw;
-    const GMEM float4 *box;    // per tile: (centre, -), (half extent, -); pairs of tiles farther apart than skip2 are skipped
-    float skip2;
+    // per chunk c: [4c] centre and [4c+1] half extents of its box in the
+    // minimum-image frame of its first particle, [4c+2] / [4c+3] lower and
+    // upper corners of its raw coordinates
+    const GMEM float4 *cbox;
+    float skip2;               // chunk pairs whose distance lower bound exceeds this are skipped
     cpos_t pos;
     const GMEM int *ptype;     // nullptr: type from pos.w
-    const GMEM float2 *thr2;   // (thr[k], thr[k+1]) k = 0..bins-1; thr = bin thresholds on d^2
-    float out_thr;             // smallest excluded d^2 (d >= min(L)/2)
-    GMEM unsigned *part;       // [gridDim][nclass*bins]
+    const GMEM float *thr;     // thr[k] k = 0..bins: smallest d^2 of bin >= k (thr[bins]: first excluded)
+    GMEM unsigned *part;       // [gridDim][nt][nt][bins]
 };
 
-// Unordered pairs i < j are visited once; each adds one count to the class
-// {ti, tj}.  Ordered counts are recovered on the host (the MIC distance is
-// symmetric bit for bit, so d(i,j) == d(j,i)).  The bin of a pair is found
-// without a square root: a fast estimate k0 from v_sqrt_f32 is corrected
-// against exact thresholds on d^2 (thr[k] = smallest float32 d^2 whose bin
-// under the reference's float64 rule is >= k), so the counts are exact.
+// Image of every pair of two chunks in one dimension.  fl(xj - xi) lies in
+// [fl(jlo - ihi), fl(jhi - ilo)] because rounding is monotone, so the
+// minimum-image rule |d| > L/2 -> d - copysign(L, d) takes one branch for
+// all of them, or the chunks are mixed: 0 = no wrap, 1 = d - L, 2 = d + L,
+// 3 = mixed.
+__device__ __forceinline__ int rdf_dim_code(float ilo, float ihi, float jlo, float jhi, float L) {
+#pragma clang fp contract(off)
+    const float a = jlo - ihi, b = jhi - ilo, h = 0.5f * L;
+    if (a >= -h && b <= h) return 0;
+    if (a > h) return 1;
+    if (b < -h) return 2;
+    return 3;
+}
+
+// Pairs of lane i with j = js..je-1 of one staged chunk.  MX bits 0..2 mark
+// mixed dimensions, which take the per-pair minimum image; every other
+// dimension takes the image its code fixes for the whole chunk pair,
+// fl(fl(xj - xi) - s) with s = L, -L or 0, which is what mic1 returns, bit
+// for bit (MX = 8: no dimension wraps, s = 0 dropped).  The bin: the
+// estimate k0 = floor(sqrt(d2) * inv_bw) with inv_bw scaled by (1 - 2^-20),
+// which outweighs the rounding of v_sqrt_f32 and of the product, so k0 is
+// the exact bin or one below it; one comparison with the exact threshold
+// thr[k0 + 1] settles it.  Excluded pairs (d2 >= thr[bins], i.e. d >= L/2)
+// are masked off: an atomic of theirs would land on one word per type pair.
+template <int MX>
+__device__ __forceinline__ void rdf_chunk(const float4 *tj, int js, int je, float xi, float yi, float zi, float sx,
+                                          float sy, float sz, float L0, float L1, float L2, float inv_bw,
+                                          float kmax, float out_thr, const float *thr, char *hist_i) {
+#pragma clang fp contract(off)
+    auto dist2 = [&](const float4 rj) {
+        float dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
+        if (MX != 8) {
+            dx = (MX & 1) ? mic1(dx, L0) : dx - sx;
+            dy = (MX & 2) ? mic1(dy, L1) : dy - sy;
+            dz = (MX & 4) ? mic1(dz, L2) : dz - sz;
+        }
+        float d2 = dx * dx;
+        d2 = d2 + dy * dy;
+        d2 = d2 + dz * dz;
+        return d2;
+    };
+    int jj = js;
+    // four pairs at a time: the threshold reads of all four are issued before
+    // the first histogram atomic (LDS reads do not pass an LDS atomic)
+    for (; jj + 4 <= je; jj += 4) {
+        float4 rj[4];
+        float d2[4], th[4];
+        int k0[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rj[u] = tj[jj + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d2[u] = dist2(rj[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) k0[u] = (int)fminf(__builtin_amdgcn_sqrtf(d2[u]) * inv_bw, kmax);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) th[u] = thr[k0[u] + 1];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (d2[u] < out_thr)
+                atomicAdd((unsigned *)(hist_i + __float_as_int(rj[u].w) + 4 * k0[u] + (d2[u] >= th[u] ? 4 : 0)), 1u);
+    }
+    for (; jj < je; ++jj) {
+        const float4 rj = tj[jj];
+        const float d2 = dist2(rj);
+        const int k0 = (int)fminf(__builtin_amdgcn_sqrtf(d2) * inv_bw, kmax);
+        if (d2 < out_thr)
+            atomicAdd((unsigned *)(hist_i + __float_as_int(rj.w) + 4 * k0 + (d2 >= thr[k0 + 1] ? 4 : 0)), 1u);
+    }
+}
+
+// Unordered pairs i < j are visited once and counted in the slot of the
+// ordered type pair (type of the row particle i, type of j); the host adds
+// the (a, b) and (b, a) slots.  A block takes tile pairs (I, J), I <= J;
+// its wavefront w takes the chunk I*4 + w against the four chunks of J,
+// each chunk pair skipped when no pair can be closer than L/2, and handled
+// by the cheapest exact distance form its image code allows.
 __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
     extern __shared__ __align__(16) char lds[];
-    const int nh = R.nclass * R.bins;
+    const int nb1 = R.bins + 1;   // thresholds
+    const int nh = R.nt * R.nt * R.bins;
     unsigned *hist = (unsigned *)lds;
-    float2 *thr = (float2 *)(lds + (((size_t)nh * 4 + 15) & ~(size_t)15));
-    float4 *tj = (float4 *)(thr + R.bins);
-    int *cls = (int *)(tj + R.tile);   // class offset table [nt*nt]
+    float *thr = (float *)(lds + (((size_t)nh * 4 + 15) & ~(size_t)15));
+    float4 *tj = (float4 *)((char *)thr + (((size_t)nb1 * 4 + 15) & ~(size_t)15));
+    int *codes = (int *)(tj + RDF_TILE);   // [16]: chunk pair (w, c) -> -1 skip, else 2-bit image codes x|y<<2|z<<4
     for (int x = threadIdx.x; x < nh; x += blockDim.x) hist[x] = 0u;
-    for (int x = threadIdx.x; x < R.bins; x += blockDim.x) thr[x] = R.thr2[x];
-    for (int x = threadIdx.x; x < R.nt * R.nt; x += blockDim.x) {
-        int a = x / R.nt, b = x % R.nt;
-        int lo = min(a, b), hi = max(a, b);
-        cls[x] = (lo * R.nt - (lo * (lo - 1)) / 2 + (hi - lo)) * R.bins;
-    }
-    __syncthreads();
-    const float out_thr = R.out_thr;
+    for (int x = threadIdx.x; x < nb1; x += blockDim.x) thr[x] = R.thr[x];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long T = R.ntiles;
-    const int bmax = R.bins - 1;
+    const float kmax = (float)(R.bins - 1), out_thr = R.thr[R.bins];
     for (long long p = blockIdx.x; p < R.npairs; p += gridDim.x) {
         // decode p -> (I, J), J >= I, rows of length T - I
         double a = (double)(2 * T + 1);
@@ -223,67 +300,78 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
         while (I > 0 && I * T - (I * (I - 1)) / 2 > p) --I;
         while (I + 1 < T && (I + 1) * T - ((I + 1) * I) / 2 <= p) ++I;
         const long long J = I + (p - (I * T - (I * (I - 1)) / 2));
-        {
-            // lower bound of the minimum-image distance between the two tiles
-            const float4 ci = R.box[2 * I], hi = R.box[2 * I + 1], cj = R.box[2 * J], hj = R.box[2 * J + 1];
-            const float gx = fmaxf(0.f, fabsf(mic1(ci.x - cj.x, R.L0)) - (hi.x + hj.x));
-            const float gy = fmaxf(0.f, fabsf(mic1(ci.y - cj.y, R.L1)) - (hi.y + hj.y));
-            const float gz = fmaxf(0.f, fabsf(mic1(ci.z - cj.z, R.L2)) - (hi.z + hj.z));
-            if (gx * gx + gy * gy + gz * gz > R.skip2) continue;   // every pair is beyond L/2
+        __syncthreads();   // the previous tile pair's chunk and codes are read
+        if (threadIdx.x < 16) {
+            const int cw = threadIdx.x >> 2, cc = threadIdx.x & 3;
+            const long long ci = I * 4 + cw, cj = J * 4 + cc;
+            int code = -1;
+            if (ci < R.nchunks && cj < R.nchunks && !(I == J && cc < cw)) {
+                const float4 bi = R.cbox[4 * ci], hi = R.cbox[4 * ci + 1], bj = R.cbox[4 * cj],
+                             hj = R.cbox[4 * cj + 1];
+                // lower bound of the minimum-image distance between the chunks
+                const float gx = fmaxf(0.f, fabsf(mic1(bi.x - bj.x, R.L0)) - (hi.x + hj.x));
+                const float gy = fmaxf(0.f, fabsf(mic1(bi.y - bj.y, R.L1)) - (hi.y + hj.y));
+                const float gz = fmaxf(0.f, fabsf(mic1(bi.z - bj.z, R.L2)) - (hi.z + hj.z));
+                if (gx * gx + gy * gy + gz * gz <= R.skip2) {
+                    const float4 li = R.cbox[4 * ci + 2], ui = R.cbox[4 * ci + 3], lj = R.cbox[4 * cj + 2],
+                                 uj = R.cbox[4 * cj + 3];
+                    code = rdf_dim_code(li.x, ui.x, lj.x, uj.x, R.L0) |
+                           (rdf_dim_code(li.y, ui.y, lj.y, uj.y, R.L1) << 2) |
+                           (rdf_dim_code(li.z, ui.z, lj.z, uj.z, R.L2) << 4);
+                }
+            }
+            codes[threadIdx.x] = code;
         }
         __syncthreads();
-        const int j0 = (int)(J * R.tile);
-        for (int x = threadIdx.x; x < R.tile; x += blockDim.x) {
-            const int j = j0 + x;
+        int live = 0;   // the same for every thread of the block
+#pragma unroll
+        for (int x = 0; x < 16; ++x) live |= codes[x] >= 0;
+        if (!live) continue;
+        const int j0 = (int)(J * RDF_TILE);
+        {
+            const int j = j0 + threadIdx.x;
             if (j < R.N) {
                 float4 rj = R.pos[j];
-                if (R.ptype) rj.w = __int_as_float(R.ptype[j]);
-                tj[x] = rj;
+                const int t = R.ptype ? R.ptype[j] : __float_as_int(rj.w);
+                rj.w = __int_as_float(t * R.bins * 4);
+                tj[threadIdx.x] = rj;
             }
         }
         __syncthreads();
-        const int i = (int)(I * R.tile) + threadIdx.x;
-        if (threadIdx.x < R.tile && i < R.N) {
+        const int i = (int)(I * RDF_TILE) + threadIdx.x;
+        if (i < R.N) {
             const float4 ri = R.pos[i];
             const int ti = R.ptype ? R.ptype[i] : __float_as_int(ri.w);
-            const int *clsi = cls + ti * R.nt;
-            const int jn = min(R.tile, R.N - j0);
-            const int jstart = (I == J) ? (threadIdx.x + 1) : 0;
-            // four pairs per iteration: the LDS reads and the dependent bin
-            // arithmetic of the four overlap
-            int jj = jstart;
-            for (; jj + 4 <= jn; jj += 4) {
-                float4 rj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) rj[u] = tj[jj + u];
-                float d2[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    float dx, dy, dz;
-                    d2[u] = dist_sq_mic(rj[u].x, rj[u].y, rj[u].z, ri.x, ri.y, ri.z, R.L0, R.L1, R.L2, dx, dy, dz);
+            char *hist_i = (char *)hist + (size_t)ti * R.nt * R.bins * 4;
+            for (int c = 0; c < 4; ++c) {
+                const int code = __builtin_amdgcn_readfirstlane(codes[w * 4 + c]);
+                if (code < 0) continue;
+                const int jb = j0 + c * RDF_CHUNK;
+                const int je = min(RDF_CHUNK, R.N - jb);
+                const int js = (I == J && c == w) ? lane + 1 : 0;
+                const float4 *tjc = tj + c * RDF_CHUNK;
+                const int cx = code & 3, cy = (code >> 2) & 3, cz = (code >> 4) & 3;
+                const float sx = cx == 1 ? R.L0 : (cx == 2 ? -R.L0 : 0.f);
+                const float sy = cy == 1 ? R.L1 : (cy == 2 ? -R.L1 : 0.f);
+                const float sz = cz == 1 ? R.L2 : (cz == 2 ? -R.L2 : 0.f);
+                const int mx = (cx == 3 ? 1 : 0) | (cy == 3 ? 2 : 0) | (cz == 3 ? 4 : 0);
+#define GMD_RDF_CHUNK(M)                                                                                        \
+    case M:                                                                                                     \
+        rdf_chunk<M>(tjc, js, je, ri.x, ri.y, ri.z, sx, sy, sz, R.L0, R.L1, R.L2, R.inv_bw, kmax, out_thr, thr, hist_i); \
+        break;
+                switch (code == 0 ? 8 : mx) {
+                    GMD_RDF_CHUNK(8)
+                    GMD_RDF_CHUNK(0)
+                    GMD_RDF_CHUNK(1)
+                    GMD_RDF_CHUNK(2)
+                    GMD_RDF_CHUNK(3)
+                    GMD_RDF_CHUNK(4)
+                    GMD_RDF_CHUNK(5)
+                    GMD_RDF_CHUNK(6)
+                    default:
+                        GMD_RDF_CHUNK(7)
                 }
-                int k[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) k[u] = min((int)(__builtin_amdgcn_sqrtf(d2[u]) * R.inv_bw), bmax);
-                float2 th[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) th[u] = thr[k[u]];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kk = k[u] - (d2[u] < th[u].x ? 1 : 0) + (d2[u] >= th[u].y ? 1 : 0);
-                    if (d2[u] < out_thr) atomicAdd(&hist[clsi[__float_as_int(rj[u].w)] + kk], 1u);
-                }
-            }
-            for (; jj < jn; ++jj) {
-                const float4 rj = tj[jj];
-                float dx, dy, dz;
-                const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, R.L0, R.L1, R.L2, dx, dy, dz);
-                if (d2 < out_thr) {
-                    int k = min((int)(__builtin_amdgcn_sqrtf(d2) * R.inv_bw), bmax);
-                    const float2 th = thr[k];
-                    k = k - (d2 < th.x ? 1 : 0) + (d2 >= th.y ? 1 : 0);
-                    atomicAdd(&hist[clsi[__float_as_int(rj.w)] + k], 1u);
-                }
+#undef GMD_RDF_CHUNK
             }
         }
     }
@@ -292,49 +380,50 @@ __global__ void __launch_bounds__(256) k_rdf(RdfP R) {
     for (int x = threadIdx.x; x < nh; x += blockDim.x) dst[x] = hist[x];
 }
 
-// Bounding box of each tile of 256 consecutive particles, in the frame of the
-// tile's first particle (minimum-image displacements), so a tile that
-// straddles the periodic boundary still gets a tight box.
-__global__ void __launch_bounds__(256) k_rdf_boxes(const float4 *pos_p, int N, float L0, float L1, float L2,
-                                                   float4 *box_p) {
+// Boxes of each chunk of 64 consecutive particles (one wavefront per chunk):
+// centre and half extents in the minimum-image frame of the chunk's first
+// particle (tight for a chunk that straddles the periodic boundary), with a
+// rounding margin, and the raw coordinate bounds.
+__global__ void __launch_bounds__(256) k_rdf_boxes(const float4 *pos_p, int N, int nchunks, float L0, float L1,
+                                                   float L2, float4 *box_p) {
     const GMEM float4 *pos = (const GMEM float4 *)pos_p;
     GMEM float4 *box = (GMEM float4 *)box_p;
-    __shared__ float red[6][4];
-    const int t = blockIdx.x, i0 = t * 256, i = i0 + threadIdx.x;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= nchunks) return;
+    const int i0 = c * RDF_CHUNK, i = i0 + lane;
     const float4 r0 = pos[i0];
-    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
-    if (i < N) {
-        const float4 r = pos[i];
-        const float d[3] = {mic1(r.x - r0.x, L0), mic1(r.y - r0.y, L1), mic1(r.z - r0.z, L2)};
-        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = d[k];
-    }
+    const float4 r = i < N ? pos[i] : r0;
+    float lo[3] = {mic1(r.x - r0.x, L0), mic1(r.y - r0.y, L1), mic1(r.z - r0.z, L2)};
+    float hi[3] = {lo[0], lo[1], lo[2]};
+    float rlo[3] = {r.x, r.y, r.z};
+    float rhi[3] = {r.x, r.y, r.z};
     for (int k = 0; k < 3; ++k) {
         for (int o = 32; o > 0; o >>= 1) {
             lo[k] = fminf(lo[k], __shfl_xor(lo[k], o, 64));
             hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o, 64));
+            rlo[k] = fminf(rlo[k], __shfl_xor(rlo[k], o, 64));
+            rhi[k] = fmaxf(rhi[k], __shfl_xor(rhi[k], o, 64));
         }
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
+    if (lane == 0) {
+        float cc[3], h[3];
         for (int k = 0; k < 3; ++k) {
-            red[k][w] = lo[k];
-            red[3 + k][w] = hi[k];
+            h[k] = 0.5f * (hi[k] - lo[k]) * 1.0001f + 1e-4f;   // rounding margin
+            cc[k] = 0.5f * (lo[k] + hi[k]);
         }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float c[3], h[3];
-        for (int k = 0; k < 3; ++k) {
-            float a = red[k][0], b = red[3 + k][0];
-            for (int x = 1; x < 4; ++x) {
-                a = fminf(a, red[k][x]);
-                b = fmaxf(b, red[3 + k][x]);
-            }
-            h[k] = 0.5f * (b - a) * 1.0001f + 1e-4f;   // rounding margin
-            c[k] = 0.5f * (a + b);
-        }
-        box[2 * t] = make_float4(r0.x + c[0], r0.y + c[1], r0.z + c[2], 0.f);
-        box[2 * t + 1] = make_float4(h[0], h[1], h[2], 0.f);
+        box[4 * c] = make_float4(r0.x + cc[0], r0.y + cc[1], r0.z + cc[2], 0.f);
+        box[4 * c + 1] = make_float4(h[0], h[1], h[2], 0.f);
+        box[4 * c + 2] = make_float4(rlo[0], rlo[1], rlo[2], 0.f);
+        box[4 * c + 3] = make_float4(rhi[0], rhi[1], rhi[2], 0.f);
     }
+}
+
+// Plain streaming copy (n float4), the known-size HBM workload of gmd_copy_probe.
+__global__ void __launch_bounds__(256) k_copy(const float4 *src_p, float4 *dst_p, size_t n) {
+    const GMEM float4 *src = (const GMEM float4 *)src_p;
+    GMEM float4 *dst = (GMEM float4 *)dst_p;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // Sum the per-block histograms: thread (x, chunk) adds 'per' blocks of bin x.
@@ -493,11 +582,13 @@ int dalloc(T **p, size_t n) {
 }
 
 typedef void (*kfun)(const DP *);
+typedef void (*sfun)(const DP *, int);
 typedef void (*bfun)(const DP *const *, const BatchBlk *);
 
 // the kernel instances of one (pair function, cutoff, integrator) combination
 struct Kernels {
-    kfun pers, step;
+    kfun pers;
+    sfun step;
     bfun bpers, bstep;
 };
 
@@ -598,9 +689,9 @@ struct gmd_ctx {
 };
 
 struct gmd_rdf {
-    int N, nt, bins, nclass;
+    int N, nt, bins, nchunks;
     float4 *pos = nullptr;
-    float4 *tbox = nullptr;   // tile boxes
+    float4 *tbox = nullptr;   // chunk boxes, 4 per chunk
     int *ptype = nullptr;
     float *d_thr = nullptr;
     unsigned *part = nullptr;
@@ -790,7 +881,7 @@ static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_st
     TRY(dalloc(&c->d_params, c->params.size()));
     TRY(dalloc(&c->ctrl, 1));
     const int max_grid = std::max(c->grid_step, c->cus * 32);
-    TRY(dalloc(&c->partial, (size_t)max_grid * GMD_NRED));
+    TRY(dalloc(&c->partial, (size_t)2 * max_grid * GMD_NRED));   // two buffers: steps alternate (folded tail)
     TRY(dalloc(&c->saver_out, (size_t)std::max(1, d.scalar_saves_per_block) * 6));
     TRY(dalloc(&c->perm, N));
     {
@@ -1181,20 +1272,17 @@ static int run_impl(gmd_ctx *c, float t0, int steps) {
     } else if (c->d.use_graph) {
         if (!c->gexec) {
             HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < GRAPH_STEPS; ++k) {
-                hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
-                hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, dP, c->d.integrator);
-            }
+            for (int k = 0; k < GRAPH_STEPS; ++k)
+                hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP, k);
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, c->ctrl, GRAPH_STEPS);
             HIPCHECK(hipStreamEndCapture(st, &c->graph));
             HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
         }
         const int launches = (steps + 1 + GRAPH_STEPS - 1) / GRAPH_STEPS;
         for (int k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(c->gexec, st));
     } else {
-        for (int s = 0; s <= steps; ++s) {
-            hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
-            hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, dP, c->d.integrator);
-        }
+        for (int s = 0; s <= steps; ++s)
+            hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP, s);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipEventRecord(c->ev1, st));
@@ -1227,7 +1315,7 @@ int gmd_profile_steps(gmd_ctx *c, float time_zero, int steps, float *step_ms, fl
     hipStream_t st = c->stream;
     const DP *dP = c->d_P;
     const int n = steps + 1;
-    std::vector<hipEvent_t> ev((size_t)3 * n, nullptr);
+    std::vector<hipEvent_t> ev((size_t)2 * n, nullptr);
     int rc = 0;
     for (auto &e : ev)
         if (hipEventCreate(&e) != hipSuccess) {
@@ -1236,26 +1324,22 @@ int gmd_profile_steps(gmd_ctx *c, float time_zero, int steps, float *step_ms, fl
         }
     if (rc == 0) {
         for (int s = 0; s < n; ++s) {
-            hipEventRecord(ev[3 * s], st);
-            hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP);
-            hipEventRecord(ev[3 * s + 1], st);
-            hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, dP, c->d.integrator);
-            hipEventRecord(ev[3 * s + 2], st);
+            hipEventRecord(ev[2 * s], st);
+            hipLaunchKernelGGL(c->kern.step, dim3(c->grid_step), dim3(c->block), c->lds_step, st, dP, s);
+            hipEventRecord(ev[2 * s + 1], st);
         }
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
             rc = fail("gmd_profile_steps: launch failed");
     }
     if (rc == 0) {
-        double a = 0.0, b = 0.0;
+        double a = 0.0;
         for (int s = 0; s < n; ++s) {
-            float x = 0.f, y = 0.f;
-            hipEventElapsedTime(&x, ev[3 * s], ev[3 * s + 1]);
-            hipEventElapsedTime(&y, ev[3 * s + 1], ev[3 * s + 2]);
+            float x = 0.f;
+            hipEventElapsedTime(&x, ev[2 * s], ev[2 * s + 1]);
             a += x;
-            b += y;
         }
         *step_ms = (float)(a / n);
-        *tail_ms = (float)(b / n);
+        *tail_ms = 0.0f;   // the tail of each step runs inside the next step kernel
     }
     for (auto &e : ev)
         if (e) hipEventDestroy(e);
@@ -1305,6 +1389,42 @@ int gmd_set_reorder_interval(gmd_ctx *c, int steps) {
     return 0;
 }
 
+int gmd_copy_probe(unsigned long long bytes, int reps, float *ms_per_copy) {
+    if (!ms_per_copy) return fail("gmd_copy_probe: null argument");
+    if (bytes == 0 || bytes % 16 || reps < 1) return fail("gmd_copy_probe: bytes must be a positive multiple of 16");
+    float4 *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
+        rc = fail("gmd_copy_probe: out of device memory");
+    } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+               hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail("gmd_copy_probe: stream or event creation failed");
+    } else {
+        const size_t n = bytes / 16;
+        int dev = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, (size_t)cus * 16));
+        bool ok = hipMemsetAsync(src, 0, bytes, st) == hipSuccess;
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, (const float4 *)src, dst, n);
+        ok = ok && hipEventRecord(e0, st) == hipSuccess;
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, (const float4 *)src, dst, n);
+        ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, st) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess;
+        float ms = 0.f;
+        if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail("gmd_copy_probe: launch failed");
+        else *ms_per_copy = ms / reps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (src) hipFree(src);
+    if (dst) hipFree(dst);
+    return rc;
+}
+
 int gmd_force_rebuild(gmd_ctx *c) {
     if (!c) return fail("gmd_force_rebuild: null context");
     c->need_rebuild = true;
@@ -1350,7 +1470,8 @@ static void rdf_thresholds(const float box[3], int bins, std::vector<float> &thr
     float mn = std::min(box[0], std::min(box[1], box[2]));
     double half = (double)mn / 2.0;
     double bw = half / bins;
-    thr.assign(bins + 1, 0.f);
+    thr.assign(bins + 2, 0.f);
+    thr[bins + 1] = INFINITY;
     for (int k = 1; k <= bins; ++k) {
         // smallest non-negative float x with bin(x) >= k (bin is monotone in x)
         uint32_t lo = 0, hi = 0x7f800000u;  // +inf
@@ -1369,9 +1490,11 @@ static void rdf_thresholds(const float box[3], int bins, std::vector<float> &thr
     }
 }
 
-static size_t rdf_lds_bytes(int nclass, int bins, int nt) {
-    const size_t nh = (size_t)nclass * bins;
-    return ((nh * 4 + 15) & ~(size_t)15) + (size_t)bins * 8 + 256 * 16 + (size_t)nt * nt * 4;
+static size_t rdf_hist_size(int nt, int bins) { return (size_t)nt * nt * bins; }
+
+static size_t rdf_lds_bytes(int nt, int bins) {
+    const size_t nh = rdf_hist_size(nt, bins);
+    return ((nh * 4 + 15) & ~(size_t)15) + (((size_t)(bins + 1) * 4 + 15) & ~(size_t)15) + RDF_TILE * 16 + 16 * 4;
 }
 
 int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
@@ -1382,23 +1505,25 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
     r->N = N;
     r->nt = ntypes;
     r->bins = bins;
-    r->nclass = ntypes * (ntypes + 1) / 2;
-    const size_t nh = (size_t)r->nclass * bins;
-    if (rdf_lds_bytes(r->nclass, bins, ntypes) > 64 * 1024) {
+    r->nchunks = (N + RDF_CHUNK - 1) / RDF_CHUNK;
+    const size_t nh = rdf_hist_size(ntypes, bins);
+    if (rdf_lds_bytes(ntypes, bins) > 64 * 1024) {
         delete r;
-        return fail("gmd_rdf_create: %d type pairs x %d bins do not fit the LDS histogram", ntypes * (ntypes + 1) / 2,
+        return fail("gmd_rdf_create: %d x %d type pairs x %d bins do not fit the LDS histogram", ntypes, ntypes,
                     bins);
     }
     int dev = 0, cus = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const long long T = (N + 255) / 256;
+    const long long T = (N + RDF_TILE - 1) / RDF_TILE;
     const long long npairs = T * (T + 1) / 2;
-    r->grid = (int)std::max(1LL, std::min(npairs, (long long)cus * 4));
+    // a block's 32-bit counters see at most 2^31 pairs
+    const long long need = (long long)(((double)N * (double)N / 2.0) / 2147483648.0) + 1;
+    r->grid = (int)std::max(1LL, std::min(npairs, std::max((long long)cus * 4, need)));
     if (hipMalloc(&r->pos, (size_t)N * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&r->tbox, (size_t)2 * T * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&r->tbox, (size_t)4 * r->nchunks * sizeof(float4)) != hipSuccess ||
         hipMalloc(&r->ptype, (size_t)N * sizeof(int)) != hipSuccess ||
-        hipMalloc(&r->d_thr, (size_t)bins * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&r->d_thr, (size_t)(bins + 1) * sizeof(float)) != hipSuccess ||
         hipMalloc(&r->part, (size_t)r->grid * nh * sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&r->d_out, nh * sizeof(unsigned long long)) != hipSuccess) {
         gmd_rdf_destroy(r);
@@ -1425,18 +1550,15 @@ int gmd_rdf_destroy(gmd_rdf *r) {
 static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3], unsigned long long *counts) {
     if (box[0] != r->box[0] || box[1] != r->box[1] || box[2] != r->box[2] || r->thr.empty()) {
         rdf_thresholds(box, r->bins, r->thr);
-        std::vector<float2> t2(r->bins);
-        for (int k = 0; k < r->bins; ++k) t2[k] = make_float2(r->thr[k], r->thr[k + 1]);
-        HIPCHECK(hipMemcpy(r->d_thr, t2.data(), t2.size() * sizeof(float2), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(r->d_thr, r->thr.data(), (size_t)(r->bins + 1) * sizeof(float), hipMemcpyHostToDevice));
         memcpy(r->box, box, sizeof r->box);
     }
     RdfP R;
     R.N = r->N;
     R.nt = r->nt;
     R.bins = r->bins;
-    R.nclass = r->nclass;
-    R.tile = 256;
-    R.ntiles = (r->N + 255) / 256;
+    R.nchunks = r->nchunks;
+    R.ntiles = (r->N + RDF_TILE - 1) / RDF_TILE;
     R.npairs = (long long)R.ntiles * (R.ntiles + 1) / 2;
     R.L0 = box[0];
     R.L1 = box[1];
@@ -1444,20 +1566,19 @@ static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const floa
     {
         float mn = std::min(box[0], std::min(box[1], box[2]));
         double bw = ((double)mn / 2.0) / r->bins;
-        R.inv_bw = (float)(1.0 / bw);
+        R.inv_bw = (float)((1.0 / bw) * (1.0 - 1.0 / 1048576.0));   // estimate never above the exact bin
     }
     R.pos = (cpos_t)pos;
     R.ptype = (const GMEM int *)d_ptype;
-    R.thr2 = (const GMEM float2 *)r->d_thr;
-    R.out_thr = r->thr[r->bins];
+    R.thr = (const GMEM float *)r->d_thr;
     R.part = (GMEM unsigned *)r->part;
-    R.box = (const GMEM float4 *)r->tbox;
-    R.skip2 = R.out_thr;
-    const size_t nh = (size_t)r->nclass * r->bins;
+    R.cbox = (const GMEM float4 *)r->tbox;
+    R.skip2 = r->thr[r->bins];
+    const size_t nh = rdf_hist_size(r->nt, r->bins);
     HIPCHECK(hipMemsetAsync(r->d_out, 0, nh * sizeof(unsigned long long), r->stream));
-    hipLaunchKernelGGL(k_rdf_boxes, dim3(R.ntiles), dim3(256), 0, r->stream, pos, r->N, box[0], box[1], box[2],
-                       r->tbox);
-    hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), rdf_lds_bytes(r->nclass, r->bins, r->nt), r->stream, R);
+    hipLaunchKernelGGL(k_rdf_boxes, dim3((unsigned)((r->nchunks + 3) / 4)), dim3(256), 0, r->stream, pos, r->N,
+                       r->nchunks, box[0], box[1], box[2], r->tbox);
+    hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), rdf_lds_bytes(r->nt, r->bins), r->stream, R);
     HIPCHECK(hipGetLastError());
     const int per = 16;
     hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256), (unsigned)((r->grid + per - 1) / per)),
@@ -1466,16 +1587,15 @@ static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const floa
     std::vector<unsigned long long> h(nh);
     HIPCHECK(hipMemcpyAsync(h.data(), r->d_out, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));
     HIPCHECK(hipStreamSynchronize(r->stream));
+    // unordered pair counts in ordered-type slots -> ordered pair counts
     const int nt = r->nt, bins = r->bins;
     for (int a = 0; a < nt; ++a)
-        for (int b = 0; b < nt; ++b) {
-            int lo = std::min(a, b), hi = std::max(a, b);
-            int cls = lo * nt - (lo * (lo - 1)) / 2 + (hi - lo);
+        for (int b = 0; b < nt; ++b)
             for (int k = 0; k < bins; ++k) {
-                unsigned long long v = h[(size_t)cls * bins + k];
-                counts[((size_t)a * nt + b) * bins + k] = (a == b) ? 2 * v : v;
+                const unsigned long long ab = h[((size_t)a * nt + b) * bins + k];
+                const unsigned long long ba = h[((size_t)b * nt + a) * bins + k];
+                counts[((size_t)a * nt + b) * bins + k] = (a == b) ? 2 * ab : ab + ba;
             }
-        }
     return 0;
 }
 

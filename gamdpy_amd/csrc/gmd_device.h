@@ -58,7 +58,9 @@ struct Ctrl {
     float t0;
     int cell_overflow;         // largest cell occupancy seen beyond the cell capacity
     long long lstep0;          // Langevin counter of step 0 of the run
-    int pad[2];
+    float xib[2];              // xi used by step s at xib[s & 1] (folded-tail path)
+    int s_base;                // step of launch argument k = 0 (folded-tail path)
+    int pad;
 };
 
 // Per-type-pair constants, prepared in LDS at kernel start.

@@ -16,10 +16,12 @@
 // particle can be integrated as soon as its own force is known.  After the
 // update each particle inserts itself into the cell grid of the next step
 // (three rotating buffers), so a rebuild never needs an extra pass or a grid
-// barrier.  The step's tail (the one-block k_tail kernel, or one block of
-// the persistent kernel after its grid barrier) finishes the deterministic
-// reductions (thermostat, momentum reset, scalar saver) and publishes the
-// rebuild decision of the next step.
+// barrier.  The step's tail finishes the deterministic reductions
+// (thermostat, momentum reset, scalar saver): in the persistent kernel one
+// block runs it after the grid barrier; in the kernel-per-step path every
+// block of the next step runs it (folded_tail); for batched replicas the
+// one-block-per-replica k_batch_tail runs it and publishes the rebuild
+// decision of the next step.
 #pragma once
 #include "gmd_device.h"
 
@@ -651,19 +653,32 @@ __device__ __forceinline__ void step_body(const DP &P, const BlockCtx &B, const 
     }
 }
 
-// Block-level reduction of the per-thread sums into partial[block] (fixed order).
-__device__ __forceinline__ void block_partials(const DP &P, const BlockCtx &B, float acc[GMD_NRED], double *red) {
+// The partial-sum components a step's tail reads: K for the thermostat,
+// momentum and mass for the momentum reset, the saver's five sums.
+__device__ __forceinline__ unsigned red_mask(bool ksum, bool mom, bool save) {
+    unsigned m = 0u;
+    if (ksum) m |= 1u << RED_K;
+    if (mom) m |= (7u << RED_PX) | (1u << RED_M);
+    if (save) m |= (1u << RED_U) | (1u << RED_W) | (1u << RED_LAP) | (1u << RED_K) | (1u << RED_FSQ);
+    return m;
+}
+
+// Block-level reduction of the per-thread sums of the components in 'mask'
+// into part[block] (fixed order).
+__device__ __forceinline__ void block_partials(const DP &P, const BlockCtx &B, float acc[GMD_NRED], double *red,
+                                               GMEM float *part, unsigned mask) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int c = 0; c < GMD_NRED; ++c) {
+        if (!((mask >> c) & 1u)) continue;
         float v = wave_sum_f(acc[c]);
         if (lane == 0) red[wid * GMD_NRED + c] = (double)v;
     }
     __syncthreads();
-    if (threadIdx.x < GMD_NRED) {
+    if (threadIdx.x < GMD_NRED && ((mask >> threadIdx.x) & 1u)) {
         double sum = 0.0;
         for (int w = 0; w < nw; ++w) sum += red[w * GMD_NRED + threadIdx.x];
-        P.partial[(size_t)threadIdx.x * P.pstride + B.bid] = (float)sum;
+        part[(size_t)threadIdx.x * P.pstride + B.bid] = (float)sum;
     }
     __syncthreads();
 }
@@ -672,43 +687,100 @@ __device__ __forceinline__ void block_partials(const DP &P, const BlockCtx &B, f
 // one-thread updates of the reference (NVT.py:226-237 thermostat,
 // momentum_reset.py:166-186 shift, scalar_saver.py:374-408 output row), the
 // rebuild decision of the next step and the step counter.
-__device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, const int nblocks) {
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+// A thread's first four per-block partials (blocks t, t+256, t+512, t+768)
+// of the components in 'mask', loaded together: one memory round trip for a
+// grid of up to 1024 blocks.
+#define RED_PRE 4
+__device__ __forceinline__ void load_partials(const DP &P, const GMEM float *part, const int nblocks, unsigned mask,
+                                              float v[RED_PRE][GMD_NRED]) {
+#pragma unroll
+    for (int u = 0; u < RED_PRE; ++u) {
+        const int b = threadIdx.x + u * blockDim.x;
+#pragma unroll
+        for (int c = 0; c < GMD_NRED; ++c)
+            v[u][c] = (((mask >> c) & 1u) && b < nblocks) ? part[(size_t)c * P.pstride + b] : 0.f;
+    }
+}
+
+// Sum of the per-block partials of the components in 'mask' over nblocks
+// blocks, in a fixed order (thread t: blocks t, t+256, ... in double; then
+// the wavefronts in index order): tot[c] in the threads that 'all' names
+// (every thread, or thread 0 only).  Every caller with the same data gets
+// the same bits.  v holds what load_partials returned.
+__device__ __forceinline__ void sum_partials(const DP &P, const GMEM float *part, const int nblocks, unsigned mask,
+                                             bool all, double *red, const float v[RED_PRE][GMD_NRED],
+                                             double tot[GMD_NRED]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const bool need = I.need_ksum || I.need_mom || I.need_save;
+    if (mask) {
+        double acc[GMD_NRED];
+#pragma unroll
+        for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.0;
+#pragma unroll
+        for (int u = 0; u < RED_PRE; ++u)
+#pragma unroll
+            for (int c = 0; c < GMD_NRED; ++c) acc[c] += (double)v[u][c];
+        for (int b = threadIdx.x + RED_PRE * blockDim.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+            for (int c = 0; c < GMD_NRED; ++c)
+                if ((mask >> c) & 1u) acc[c] += (double)part[(size_t)c * P.pstride + b];
+        }
+#pragma unroll
+        for (int c = 0; c < GMD_NRED; ++c) {
+            if (!((mask >> c) & 1u)) continue;
+            double x = wave_sum_d(acc[c]);
+            if (lane == 0) red[wid * GMD_NRED + c] = x;
+        }
+    }
+    __syncthreads();
+    if (all || threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < GMD_NRED; ++c) {
+            tot[c] = 0.0;
+            if ((mask >> c) & 1u)
+                for (int w = 0; w < nw; ++w) tot[c] += red[w * GMD_NRED + c];
+        }
+    }
+    __syncthreads();
+}
+
+// NVT.py:226-237: the thermostat variable after a step with kinetic sum ksum.
+__device__ __forceinline__ float thermostat_next(const DP &P, float xi, float ksum, float t0, int s) {
+    const double t = (double)t0 + (double)s * (double)P.dt;
+    const double T = target_temperature(P, t);
+    const double dev = (double)(2.0f * ksum) / (P.degrees * T) - 1.0;
+    const double inc = (double)(P.dt * P.omega2) * dev;
+    return (float)((double)xi + inc);
+}
+
+// scalar_saver.py:374-408 output row of step s
+__device__ __forceinline__ void saver_row(const DP &P, int s, const double tot[GMD_NRED]) {
+    GMEM float *row = P.saver_out + (size_t)(s / P.save_every) * 6;
+    row[0] = (float)tot[RED_U];
+    row[1] = (float)tot[RED_W];
+    row[2] = (float)tot[RED_LAP];
+    row[3] = (float)tot[RED_K];
+    row[4] = (float)tot[RED_FSQ];
+    row[5] = P.box0 * P.box1 * P.box2;
+}
+
+__device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, const int nblocks) {
     GMEM Ctrl *C = P.ctrl;
     // the rebuild flag thread 0 publishes at the end is loaded first, so the
     // load overlaps with the reduction (rebuild_flag holds nsteps+2 entries)
     int next_flag = 0;
     if (threadIdx.x == 0) next_flag = ld_flag(P.rebuild_flag + I.s + 1);
-    if (need) {
-        double acc[GMD_NRED];
-#pragma unroll
-        for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-#pragma unroll
-            for (int c = 0; c < GMD_NRED; ++c) acc[c] += (double)P.partial[(size_t)c * P.pstride + b];
-        }
-#pragma unroll
-        for (int c = 0; c < GMD_NRED; ++c) {
-            double v = wave_sum_d(acc[c]);
-            if (lane == 0) red[wid * GMD_NRED + c] = v;
-        }
-    }
-    __syncthreads();
+    const unsigned mask = red_mask(I.need_ksum, I.need_mom, I.need_save);
+    float v[RED_PRE][GMD_NRED];
+    load_partials(P, P.partial, nblocks, mask, v);
+    double tot[GMD_NRED];
+    sum_partials(P, P.partial, nblocks, mask, false, red, v, tot);
     if (threadIdx.x == 0) {
-        double tot[GMD_NRED];
-        for (int c = 0; c < GMD_NRED; ++c) {
-            tot[c] = 0.0;
-            if (need)
-                for (int w = 0; w < nw; ++w) tot[c] += red[w * GMD_NRED + c];
-        }
         if (I.need_ksum) {
-            const float ksum = (float)tot[RED_K];
-            const double t = (double)I.t0 + (double)I.s * (double)P.dt;
-            const double T = target_temperature(P, t);
-            const double dev = (double)(2.0f * ksum) / (P.degrees * T) - 1.0;
-            const double inc = (double)(P.dt * P.omega2) * dev;
-            C->xi = (float)((double)I.xi + inc);
+            C->xi = thermostat_next(P, I.xi, (float)tot[RED_K], I.t0, I.s);
             C->ksum = 0.0f;
         }
         if (I.need_mom) {
@@ -718,20 +790,74 @@ __device__ inline void step_tail(const DP &P, const StepInfo &I, double *red, co
             C->shift[2] = (float)tot[RED_PX + 2] / mt;
             C->shift_step = I.s + 1;
         }
-        if (I.need_save) {
-            GMEM float *row = P.saver_out + (size_t)(I.s / P.save_every) * 6;
-            row[0] = (float)tot[RED_U];
-            row[1] = (float)tot[RED_W];
-            row[2] = (float)tot[RED_LAP];
-            row[3] = (float)tot[RED_K];
-            row[4] = (float)tot[RED_FSQ];
-            row[5] = P.box0 * P.box1 * P.box2;
-        }
+        if (I.need_save) saver_row(P, I.s, tot);
         if (I.rebuild) C->nbflag[2] += 1;
         C->rebuild_now = (I.s < I.nsteps) ? next_flag : 0;
         C->step = I.s + 1;
     }
     __syncthreads();
+}
+
+// The tail of step q = I.s - 1 folded into step I.s (kernel-per-step path of
+// one system): every block sums step q's partials (buffer q & 1) itself, in
+// the order step_tail uses, so all blocks hold the same thermostat variable
+// and momentum shift, bit for bit, without a separate one-block launch and
+// its round trips through memory.  Block 0 publishes what the host reads
+// (thermostat state, pending shift, saver row of step q).
+// folded_tail_load issues the loads (before the pair table is staged, so the
+// two overlap); folded_tail finishes.
+template <int INTEG>
+__device__ __forceinline__ unsigned folded_tail_load(const DP &P, const StepInfo &I, float v[RED_PRE][GMD_NRED]) {
+    unsigned mask = 0u;
+    if (I.s > 0) {
+        const int q = I.s - 1;   // q < nsteps: step q integrated
+        const bool integ_q = INTEG != 0;
+        mask = red_mask(integ_q && INTEG == 2, integ_q && P.mom_every > 0 && (q % P.mom_every) == 0,
+                        integ_q && P.save_every > 0 && (q % P.save_every) == 0 && (q / P.save_every) < P.saves_per_block);
+        load_partials(P, P.partial + (size_t)(q & 1) * GMD_NRED * P.pstride, P.step_grid, mask, v);
+    }
+    return mask;
+}
+
+template <int INTEG>
+__device__ __forceinline__ void folded_tail(const DP &P, const BlockCtx &B, StepInfo &I, double *red, unsigned mask,
+                                            const float v[RED_PRE][GMD_NRED]) {
+    GMEM Ctrl *C = P.ctrl;
+    const bool pub = B.bid == 0 && threadIdx.x == 0;
+    if (I.s == 0) {
+        // xi of step 0 is the run's starting state; no shift is pending
+        I.apply_shift = false;
+        if (pub) C->xib[0] = I.xi;
+        return;
+    }
+    const int q = I.s - 1;
+    const bool ksum_q = (mask >> RED_K) & 1u && INTEG == 2;
+    const bool mom_q = (mask >> RED_M) & 1u;
+    const bool save_q = (mask >> RED_FSQ) & 1u;
+    const float xi_q = C->xib[q & 1];
+    double tot[GMD_NRED];
+    sum_partials(P, P.partial + (size_t)(q & 1) * GMD_NRED * P.pstride, P.step_grid, mask, true, red, v, tot);
+    // the same value in every lane: kept in scalar registers over the step
+    I.xi = uniform_f(ksum_q ? thermostat_next(P, xi_q, (float)tot[RED_K], I.t0, q) : xi_q);
+    I.apply_shift = mom_q;
+    if (mom_q) {
+        const float mt = (float)tot[RED_M];
+        I.sh0 = uniform_f((float)tot[RED_PX] / mt);
+        I.sh1 = uniform_f((float)tot[RED_PX + 1] / mt);
+        I.sh2 = uniform_f((float)tot[RED_PX + 2] / mt);
+    }
+    if (pub) {
+        C->xib[I.s & 1] = I.xi;
+        C->xi = I.xi;
+        if (ksum_q) C->ksum = 0.0f;
+        if (mom_q) {
+            C->shift[0] = I.sh0;
+            C->shift[1] = I.sh1;
+            C->shift[2] = I.sh2;
+            C->shift_step = I.s;
+        }
+        if (save_q) saver_row(P, q, tot);
+    }
 }
 
 // Software grid barrier for the cooperative (co-resident) launch.  The last

@@ -126,9 +126,10 @@ int gmd_run(gmd_ctx *ctx, float time_zero, int steps);
 int gmd_evaluate(gmd_ctx *ctx);
 /* GPU milliseconds of the last gmd_run (hipEvents around the launch) */
 int gmd_last_run_ms(gmd_ctx *ctx, float *ms);
-/* gmd_run with one launch per kernel and hipEvents around each: the mean
- * duration of the step kernel and of the step-tail kernel over the steps+1
- * evaluations (a measurement aid; no counterpart in the reference) */
+/* gmd_run with one launch per step kernel and hipEvents around each: the
+ * mean duration of the step kernel over the steps+1 evaluations; tail_ms is
+ * 0 (each step's cross-block tail runs inside the next step kernel).  A
+ * measurement aid; no counterpart in the reference. */
 int gmd_profile_steps(gmd_ctx *ctx, float time_zero, int steps, float *step_ms, float *tail_ms);
 
 /* NbList.d_nbflag (nblist.py:9): [update pending, overflow count, #rebuilds] */
@@ -142,6 +143,12 @@ int gmd_force_rebuild(gmd_ctx *ctx);
  * every host array keeps the caller's order.  Linked-list contexts without
  * exclusions start at 400; no counterpart in the reference. */
 int gmd_set_reorder_interval(gmd_ctx *ctx, int steps);
+/* HBM probe: a plain float4 copy kernel between two fresh device buffers of
+ * 'bytes' bytes (a multiple of 16), run 'reps' times after one untimed copy;
+ * mean milliseconds per copy.  The known-size workload against which HBM
+ * byte counters are scaled (a measurement aid; no counterpart in the
+ * reference). */
+int gmd_copy_probe(unsigned long long bytes, int reps, float *ms_per_copy);
 
 /* NVT.thermostat_state (NVT.py:143): [xi, sum K] */
 int gmd_get_thermostat(gmd_ctx *ctx, float state[2]);
