@@ -36,8 +36,10 @@ SIG = [[1.00, 0.80], [0.80, 0.88]]
 EPS = [[1.00, 1.50], [1.50, 0.50]]
 METRIC = "particle-steps/s KA-LJ N=65536; batched-replica & adjoint speedup vs plain"
 PROFILE_STEPS = 200
-# HBM bytes per k_step launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (None until measured)
-TRAFFIC_BYTES = None
+# HBM bytes per k_step launch of this workload: rocprofv3 FETCH_SIZE + WRITE_SIZE
+# over the profiled launches, each scaled by its reading for a copy kernel of
+# known size (profiles/k_step_ka65536_pmc.md)
+TRAFFIC_BYTES = 5.389e7
 
 
 def log(*a):

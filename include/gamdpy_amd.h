@@ -126,9 +126,9 @@ int gmd_run(gmd_ctx *ctx, float time_zero, int steps);
 int gmd_evaluate(gmd_ctx *ctx);
 /* GPU milliseconds of the last gmd_run (hipEvents around the launch) */
 int gmd_last_run_ms(gmd_ctx *ctx, float *ms);
-/* gmd_run with one launch per step kernel and hipEvents around each: the
- * mean duration of the step kernel over the steps+1 evaluations; tail_ms is
- * 0 (each step's cross-block tail runs inside the next step kernel).  A
+/* gmd_run with the steps+1 step kernels launched back to back between two
+ * hipEvents: their mean duration (dispatch gaps included); tail_ms is 0
+ * (each step's cross-block tail runs inside the next step kernel).  A
  * measurement aid; no counterpart in the reference. */
 int gmd_profile_steps(gmd_ctx *ctx, float time_zero, int steps, float *step_ms, float *tail_ms);
 
