@@ -254,14 +254,18 @@ __device__ __forceinline__ void rdf_chunk(const float4 *tj, int js, int je, floa
         for (int u = 0; u < 4; ++u) rj[u] = tj[jj + u];
 #pragma unroll
         for (int u = 0; u < 4; ++u) d2[u] = dist2(rj[u]);
+        // a group of four excluded in every lane skips the binning
+        if (d2[0] < out_thr || d2[1] < out_thr || d2[2] < out_thr || d2[3] < out_thr) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) k0[u] = (int)fminf(__builtin_amdgcn_sqrtf(d2[u]) * inv_bw, kmax);
+            for (int u = 0; u < 4; ++u) k0[u] = (int)fminf(__builtin_amdgcn_sqrtf(d2[u]) * inv_bw, kmax);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) th[u] = thr[k0[u] + 1];
+            for (int u = 0; u < 4; ++u) th[u] = thr[k0[u] + 1];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (d2[u] < out_thr)
-                atomicAdd((unsigned *)(hist_i + __float_as_int(rj[u].w) + 4 * k0[u] + (d2[u] >= th[u] ? 4 : 0)), 1u);
+            for (int u = 0; u < 4; ++u)
+                if (d2[u] < out_thr)
+                    atomicAdd((unsigned *)(hist_i + __float_as_int(rj[u].w) + 4 * k0[u] + (d2[u] >= th[u] ? 4 : 0)),
+                              1u);
+        }
     }
     for (; jj < je; ++jj) {
         const float4 rj = tj[jj];
