@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-replica and adjoint measurements")
     ap.add_argument("--replicas", type=int, default=128, help="replicas of the batched measurement")
     ap.add_argument("--replica-steps", type=int, default=1000, help="steps of the batched measurement")
+    ap.add_argument("--large", action="store_true",
+                    help="instead of the headline: single-component LJ, N=1048576 (FCC 64^3, rho 0.973), "
+                         "NVT_Langevin T=0.7 alpha=0.1, linked lists skin 0.5, hipGraph path")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel that reads MIB MiB and writes MIB MiB (untimed), "
                          "the known-size reference for HBM byte counters")
@@ -98,6 +101,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.large:
+        print(json.dumps(large_system(args)), flush=True)
+        return
 
     conf, pp = make_system()
     N = conf.N
@@ -214,6 +220,61 @@ def main():
     }
     out.update(extras)
     print(json.dumps(out), flush=True)
+
+
+def large_system(args):
+    """One large system: LJ (eps = sigma = 1, SP cutoff 2.5), FCC 64^3 at
+    rho = 0.973 (N = 1048576), NVT_Langevin T = 0.7, alpha = 0.1, dt = 0.005,
+    linked lists with skin 0.5, one k_step launch per step from a hipGraph.
+    Equilibrated --warmup steps, then --steps timed; the step kernel's mean
+    duration over 200 back-to-back launches gives its HBM roofline."""
+    import gamdpy_amd as gp
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[64, 64, 64], rho=0.973)
+    conf['m'] = 1.0
+    conf.randomize_velocities(temperature=0.7, seed=0)
+    N = conf.N
+    func = gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon)
+    pp = gp.PairPotential(func, params=[1.0, 1.0, 2.5], max_num_nbs=256)
+    plan = gp.get_default_compute_plan(conf)
+    plan['skin'] = 0.5
+    plan['nblist'] = 'linked lists'
+    if args.tp:
+        plan['tp'] = args.tp
+        plan['pb'] = max(1, 256 // args.tp)
+    integ = gp.NVT_Langevin(temperature=0.7, alpha=0.1, dt=0.005, seed=2024)
+    steps, warm = args.steps, args.warmup
+    sim = gp.Simulation(conf, pp, integ, [gp.MomentumReset(100)], num_timeblocks=1, steps_per_timeblock=steps,
+                        compute_plan=dict(plan), storage='memory')
+    log(f"[bench] large: N={N} plan={plan}")
+    sim.integrate_self(0.0, warm)
+    ctx = sim.ctx
+    nb0 = ctx.nbflag()[2]
+    t0 = time.perf_counter()
+    sim.integrate_self(np.float32(warm * 0.005), steps)
+    elapsed = time.perf_counter() - t0
+    gpu_ms = ctx.last_run_ms()
+    rebuilds = int(ctx.nbflag()[2] - nb0)
+    k_ms, _ = ctx.profile_steps(np.float32((warm + steps) * 0.005), PROFILE_STEPS)
+    conf.copy_to_host()
+    mean_nb, max_nb = list_length_stats(conf, pp)
+    B_ps = 4.0 * mean_nb + 92.0
+    achieved = B_ps * N / (k_ms * 1e-3) / 1e9
+    T = 2 * float(np.sum(conf['K'])) / (3 * N - 3)
+    value = N * steps / (gpu_ms * 1e-3)
+    log(f"[bench] large: {steps} steps, GPU {gpu_ms:.1f} ms ({elapsed:.3f} s wall): {value:.3e} particle-steps/s; "
+        f"k_step {k_ms * 1e3:.1f} us; rebuilds {rebuilds}; mean list {mean_nb:.1f}; T={T:.3f}")
+    return {"metric": "particle-steps/s LJ N=1048576 NVT_Langevin", "value": value, "unit": "particle-steps/s",
+            "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": gpu_ms / steps, "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "LJ SP cutoff 2.5, FCC 64^3 rho=0.973, NVT_Langevin T=0.7 alpha=0.1 dt=0.005, "
+                                   "linked lists skin 0.5, hipGraph", "N": N,
+                       "plan": {k: plan[k] for k in ('pb', 'tp', 'gridsync', 'nblist')}},
+            "nblist_rebuilds_timed": rebuilds, "mean_list_length": mean_nb, "temperature": T,
+            "kernel_profile": {"k_step_us": k_ms * 1e3, "steps": PROFILE_STEPS},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_particle_step": B_ps,
+                         "units_per_launch": N, "kernel": "k_step<LJ_sigma_epsilon,SP,NVT_Langevin>"}}
 
 
 def copy_calibration(mib):

@@ -949,6 +949,9 @@ static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_st
     P.box0 = d.box[0];
     P.box1 = d.box[1];
     P.box2 = d.box[2];
+    P.ibox0 = 1.0f / d.box[0];
+    P.ibox1 = 1.0f / d.box[1];
+    P.ibox2 = 1.0f / d.box[2];
     {
         float cps = d.max_cut + d.skin;
         P.cps2 = cps * cps;

@@ -25,10 +25,10 @@
 #pragma once
 #include "gmd_device.h"
 
-#define LL_PER 9                 // candidates per lane and pass of the cell walk
+#define LL_PER 8                 // candidates per lane and pass of the cell walk
 #define LL_IDXCAP (64 * LL_PER)  // stencil members per wave and pass
 #define LL_OWNCAP 64             // particles of the centre cell (cell capacity limit)
-#define LL_WAVE_LDS (LL_OWNCAP * 4 + LL_IDXCAP * 16)
+#define LL_WAVE_LDS (LL_OWNCAP * 4 + LL_IDXCAP * 16 + LL_IDXCAP * 2)
 #define LL_MAX_N (1 << 24)       // candidate tags pack index | type << 24
 
 typedef const GMEM float4 *cpos_t;
@@ -298,13 +298,14 @@ __device__ __forceinline__ void pair_accum(const float d2, const float dx, const
 // Interactions from the neighbour list + integration of particle i by the tp
 // lanes of its group (lane t).  All tp lanes must call it together.
 // ---------------------------------------------------------------------------
-template <int BASE, int CUT, int INTEG>
+template <int BASE, int CUT, int INTEG, bool LAP>
 __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, const PairP *pp, const int i,
                                               const int t, float acc[GMD_NRED]) {
     const int tp = P.tp;
     const int N = P.N;
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
+    const float iL0 = P.ibox0, iL1 = P.ibox1, iL2 = P.ibox2;
     // the first batch of list entries is requested before the count arrives
     // (the list is zero-initialised, so entries past the count are valid indices)
     const GMEM int *nbrow = P.nbl + (size_t)i * tp + t;
@@ -317,7 +318,6 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
     const int cnt = min(P.nbc[i], P.max_nbs);
     const int ti = __float_as_int(ri.w);
     const PairP *ppi = pp + ti * P.ntypes;
-    const bool lap = P.compute_lap != 0;
     float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
     // this lane handles entries t, t+tp, ...; UNR gathers in flight, next
     // batch of list entries prefetched while the current batch is used
@@ -332,9 +332,10 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             float dx, dy, dz;
-            const float d2 = dist_sq_mic(rj[u].x, rj[u].y, rj[u].z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
+            const float d2 =
+                dist_sq_list(rj[u].x, rj[u].y, rj[u].z, ri.x, ri.y, ri.z, L0, L1, L2, iL0, iL1, iL2, dx, dy, dz);
             const PairP &q = ppi[__float_as_int(rj[u].w)];
-            if (m + u < nloc && d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
+            if (m + u < nloc && d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, LAP);
         }
     }
     for (int mm = 1; mm < tp; mm <<= 1) {
@@ -343,7 +344,7 @@ __device__ __forceinline__ void particle_step(const DP &P, const StepInfo &I, co
         fz += __shfl_xor(fz, mm, 64);
         uu += __shfl_xor(uu, mm, 64);
         ww += __shfl_xor(ww, mm, 64);
-        ll += __shfl_xor(ll, mm, 64);
+        if (LAP) ll += __shfl_xor(ll, mm, 64);
     }
     const float4 v4 = P.vel[i];
     const float4 rr = P.rref[i];
@@ -359,8 +360,14 @@ __device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const
     const int t = threadIdx.x & (P.tp - 1);
     const int slot = threadIdx.x >> P.tp_shift;
     const int slots = blockDim.x >> P.tp_shift;
-    for (int i = B.bid * slots + slot; i < P.N; i += B.nb * slots)
-        particle_step<BASE, CUT, INTEG>(P, I, pp, i, t, acc);
+    // the laplacian sum only when asked for: a loop of its own without it
+    if (P.compute_lap) {
+        for (int i = B.bid * slots + slot; i < P.N; i += B.nb * slots)
+            particle_step<BASE, CUT, INTEG, true>(P, I, pp, i, t, acc);
+    } else {
+        for (int i = B.bid * slots + slot; i < P.N; i += B.nb * slots)
+            particle_step<BASE, CUT, INTEG, false>(P, I, pp, i, t, acc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -373,44 +380,75 @@ __device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const
 // holds the totals of particle q and integrates it.
 // ---------------------------------------------------------------------------
 // Test the staged candidates cand[0, ce) against each particle of the cell:
-// cand holds (x, y, z, index | type << 24) per candidate.
+// cand holds (x, y, z, index | type << 24) per candidate.  For particle q the
+// 64 lanes first filter the candidates (list test and list write, ballot
+// order), recording the accepted candidates' slots in accs; then they
+// evaluate q's pair terms over the accepted ones only, every lane busy,
+// instead of carrying the force through the filter where most lanes of a
+// 64-candidate batch hold rejected pairs.
 template <int BASE, int CUT>
-__device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const float4 *cand, const int ce,
-                                          const int np, const int myi, const float4 myr, int &mycount,
+__device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const float4 *cand, unsigned short *accs,
+                                          const int ce, const int np, const int myi, const float4 myr, int &mycount,
                                           float q6[6]) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
     const bool lap = P.compute_lap != 0;
+    // the descriptor fields the filter needs, held in registers (the list
+    // stores would otherwise make the compiler reload them every batch)
+    const int N = P.N, tp = P.tp, tsh = P.tp_shift, maxnb = P.max_nbs;
+    const bool niii = P.niii != 0, has_excl = P.excl != nullptr;
+    const float cps2 = P.cps2;
+    GMEM int *const nbl = P.nbl;
+    const size_t rs = (size_t)N * (size_t)tp;   // list row stride
     for (int q = 0; q < np; ++q) {
         const int iq = __shfl(myi, q, 64);
         const float rx = __shfl(myr.x, q, 64), ry = __shfl(myr.y, q, 64), rz = __shfl(myr.z, q, 64);
         const int tq = __float_as_int(__shfl(myr.w, q, 64));
         const PairP *ppq = pp + tq * P.ntypes;
+        GMEM int *const nbq = nbl + (size_t)iq * tp;
         int cntq = __shfl(mycount, q, 64);
-        float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
+        int nacc = 0;
         for (int kk = 0; kk < ce; kk += 64) {
             bool ok = false;
             int j = -1;
             if (kk + lane < ce) {
                 const float4 cj = cand[kk + lane];
-                const int tagj = __float_as_int(cj.w);
-                j = tagj & 0xffffff;
-                if (pair_accept(P, iq, j)) {
+                j = __float_as_int(cj.w) & 0xffffff;
+                bool acc;
+                if (niii) {
+                    const int two_dn = 2 * (j - iq);   // nblist.py:112-116
+                    acc = (two_dn > 0 && two_dn <= N) || (two_dn < -N);
+                } else {
+                    acc = j != iq;
+                }
+                if (acc) {
                     float dx, dy, dz;
                     const float d2 = dist_sq_mic(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, dx, dy, dz);
-                    ok = d2 < P.cps2 && !excluded(P, iq, j);
-                    const PairP &qq = ppq[tagj >> 24];
-                    if (ok && d2 < qq.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, qq, fx, fy, fz, uu, ww, ll, lap);
+                    ok = d2 < cps2 && !(has_excl && excluded(P, iq, j));
                 }
             }
             const unsigned long long mask = __ballot(ok);
             if (ok) {
-                const int slot = cntq + __popcll(mask & lt_mask);
-                if (slot < P.max_nbs) P.nbl[nbl_index(P, iq, slot)] = j;
+                const int pre = __popcll(mask & lt_mask);
+                const int slot = cntq + pre;
+                if (slot < maxnb) nbq[(size_t)(slot >> tsh) * rs + (slot & (tp - 1))] = j;
+                accs[nacc + pre] = (unsigned short)(kk + lane);
             }
-            cntq += __popcll(mask);
+            const int na = __popcll(mask);
+            cntq += na;
+            nacc += na;
         }
+        wave_sync_lds();
+        float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
+        for (int e = lane; e < nacc; e += 64) {
+            const float4 cj = cand[accs[e]];
+            float dx, dy, dz;
+            const float d2 = dist_sq_mic(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, dx, dy, dz);
+            const PairP &qq = ppq[__float_as_int(cj.w) >> 24];
+            if (d2 < qq.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, qq, fx, fy, fz, uu, ww, ll, lap);
+        }
+        wave_sync_lds();   // accs is rewritten for the next particle
         fx = wave_sum_f(fx);
         fy = wave_sum_f(fy);
         fz = wave_sum_f(fz);
@@ -472,11 +510,12 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
                                        float acc[GMD_NRED]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    // per wave: own[LL_OWNCAP] | cand[LL_IDXCAP] float4; the sorted member
+    // per wave: own[LL_OWNCAP] | cand[LL_IDXCAP] float4 | accs[LL_IDXCAP] u16; the sorted member
     // indices of a pass are staged at the start of cand, then overwritten by
     // the candidates themselves
     int *own = (int *)(lds + (size_t)wid * LL_WAVE_LDS);
     float4 *cand = (float4 *)(own + LL_OWNCAP);
+    unsigned short *accs = (unsigned short *)(cand + LL_IDXCAP);
     int *idx = (int *)cand;
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
     const GMEM int *ccount = P.ccount + (size_t)I.cbuf * P.ncells;
@@ -538,16 +577,23 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
 #pragma unroll
             for (int u = 0; u < LL_PER; ++u) jj[u] = (lane + 64 * u < ce) ? idx[lane + 64 * u] : 0;
             wave_sync_lds();
+            // the gathers in flight four at a time (an index past ce is 0, a
+            // valid particle), then the LDS writes
 #pragma unroll
-            for (int u = 0; u < LL_PER; ++u) {
-                if (64 * u < ce) {
-                    const float4 r = pos[jj[u]];
-                    const int tag = jj[u] | (__float_as_int(r.w) << 24);
-                    if (lane + 64 * u < ce) cand[lane + 64 * u] = make_float4(r.x, r.y, r.z, __int_as_float(tag));
+            for (int u0 = 0; u0 < LL_PER; u0 += 4) {
+                if (64 * u0 >= ce) break;
+                float4 rg[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rg[u] = pos[jj[u0 + u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int tag = jj[u0 + u] | (__float_as_int(rg[u].w) << 24);
+                    if (lane + 64 * (u0 + u) < ce)
+                        cand[lane + 64 * (u0 + u)] = make_float4(rg[u].x, rg[u].y, rg[u].z, __int_as_float(tag));
                 }
             }
             wave_sync_lds();
-            cell_test<BASE, CUT>(P, pp, cand, ce, np, myi, myr, mycount, q6);
+            cell_test<BASE, CUT>(P, pp, cand, accs, ce, np, myi, myr, mycount, q6);
             done = pe;
         }
         if (myi >= 0) {

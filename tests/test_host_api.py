@@ -98,7 +98,7 @@ def test_unknown_nblist_raises_value_error():
 def test_default_compute_plan_shape():
     """get_default_compute_plan.py:8-89 keys; tp fills 256 CUs x 16 waves."""
     for n, tp_expected, nbl in [(2048, 16, 'N squared'), (65536, 4, 'linked lists'),
-                                (1048576, 1, 'linked lists')]:
+                                (1048576, 2, 'linked lists')]:
         conf = gp.Configuration(D=3, N=n)
         conf.simbox = gp.Orthorhombic(3, [100.0, 100.0, 100.0])
         p = gp.get_default_compute_plan(conf)
