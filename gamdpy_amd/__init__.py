@@ -21,8 +21,8 @@ from . import configuration as _configuration_module
 from .configuration import Configuration, colarray, make_lattice, unit_cells, generate_random_velocities
 from .configuration import get_default_compute_flags
 from .simulation_boxes import Orthorhombic
-from .potential_functions import (LJ_12_6, LJ_12_6_sigma_epsilon, apply_shifted_potential_cutoff,
-                                  apply_shifted_force_cutoff)
+from .potential_functions import (LJ_12_6, LJ_12_6_sigma_epsilon, make_IPL_n, make_LJ_m_n, harmonic_repulsion,
+                                  hertzian, apply_shifted_potential_cutoff, apply_shifted_force_cutoff)
 from .interactions import Interaction, PairPotential, NbList2, NbListLinkedLists
 from . import integrators
 from .integrators import Integrator, NVE, NVT, NVT_Langevin

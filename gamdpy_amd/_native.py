@@ -15,10 +15,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgamdpy_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gamdpy_amd.h")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # enums of gamdpy_amd.h
-PAIR_BASE = {"LJ_12_6_sigma_epsilon": 0, "LJ_12_6": 1}
+PAIR_BASE = {"LJ_12_6_sigma_epsilon": 0, "LJ_12_6": 1, "IPL_n": 2, "LJ_m_n": 3, "harmonic_repulsion": 4,
+             "hertzian": 5}
 CUTOFF = {None: 0, "SP": 1, "SF": 2}
 NBLIST = {"N squared": 0, "linked lists": 1}
 INTEGRATOR = {None: 0, "NVE": 1, "NVT": 2, "NVT_Langevin": 3}
@@ -38,6 +39,7 @@ class GmdDesc(ctypes.Structure):
         ("box", ctypes.c_float * 3),
         ("pair_base", ctypes.c_int), ("cutoff", ctypes.c_int),
         ("params", c_float_p),
+        ("pair_exps", ctypes.c_float * 2),
         ("max_cut", ctypes.c_float),
         ("nblist_kind", ctypes.c_int),
         ("skin", ctypes.c_float),
@@ -210,6 +212,8 @@ def fill_desc(d, desc_fields, params, exclusions=None):
             d.box = (ctypes.c_float * 3)(*[float(x) for x in v])
         elif k == "tparams":
             d.tparams = (ctypes.c_float * 4)(*[float(x) for x in v])
+        elif k == "pair_exps":
+            d.pair_exps = (ctypes.c_float * 2)(*[float(x) for x in v])
         else:
             setattr(d, k, v)
     d.params = fptr(params)

@@ -621,7 +621,10 @@ Kernels pick_cut(int cut, int integ) {
     }
 }
 
-Kernels pick_kernels(int base, int cut, int integ) { return base == 0 ? pick_cut<0>(cut, integ) : pick_cut<1>(cut, integ); }
+// bases 2..5 share one instance (a uniform switch on the kind inside)
+Kernels pick_kernels(int base, int cut, int integ) {
+    return base == 0 ? pick_cut<0>(cut, integ) : base == 1 ? pick_cut<1>(cut, integ) : pick_cut<2>(cut, integ);
+}
 
 float i2f(int v) {
     float f;
@@ -764,7 +767,13 @@ static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_st
     if (d.D != 3) return fail("gmd_create: only D=3 is supported (got D=%d)", d.D);
     if (d.N < 1) return fail("gmd_create: N must be positive (got %d)", d.N);
     if (d.ntypes < 1 || d.ntypes > MAX_TYPES) return fail("gmd_create: ntypes must be in [1,%d]", MAX_TYPES);
-    if (d.pair_base < 0 || d.pair_base > 1) return fail("gmd_create: unknown pair function %d", d.pair_base);
+    if (d.pair_base < 0 || d.pair_base > GMD_HERTZIAN) return fail("gmd_create: unknown pair function %d", d.pair_base);
+    if ((d.pair_base == GMD_IPL_N || d.pair_base == GMD_LJ_M_N) && !(d.pair_exps[0] > 0.0f))
+        return fail("gmd_create: the exponent of IPL_n / LJ_m_n must be positive");
+    if (d.pair_base == GMD_LJ_M_N && !(d.pair_exps[1] > 0.0f))
+        return fail("gmd_create: the exponents of LJ_m_n must be positive");
+    if (d.pair_base == GMD_HERTZIAN && !(d.pair_exps[0] > 0.0f))
+        return fail("gmd_create: the hertzian exponent alpha must be positive");
     if (d.cutoff < 0 || d.cutoff > 2) return fail("gmd_create: unknown cutoff wrapper %d", d.cutoff);
     if (d.nblist_kind < 0 || d.nblist_kind > 1) return fail("gmd_create: unknown nblist kind %d", d.nblist_kind);
     if (d.integrator < 0 || d.integrator > 3) return fail("gmd_create: unknown integrator %d", d.integrator);
@@ -979,6 +988,8 @@ static int ctx_create(const gmd_desc *desc, gmd_ctx **out, hipStream_t shared_st
     P.raw_params = (const GMEM float *)c->d_params;
     P.pair_base = d.pair_base;
     P.cutoff = d.cutoff;
+    P.pexp0 = d.pair_exps[0];
+    P.pexp1 = d.pair_exps[1];
     P.dt = d.dt;
     P.omega2 = d.omega2;
     P.alpha = d.alpha;

@@ -66,9 +66,15 @@ struct Ctrl {
 // Per-type-pair constants, prepared in LDS at kernel start.
 //   LJ sigma/epsilon: a = sigma^2, b = 4 eps, c = 24 eps
 //   LJ 12-6:          a = A12, b = A6, c = 12 A12, d = 6 A6
+//   IPL_n:            a = A_n, c = n A_n, e0 = n
+//   LJ_m_n:           a = A_m, b = A_n, c = m A_m, d = n A_n, e0 = m, e1 = n
+//   harmonic:         a = eps, b = sigma
+//   hertzian:         a = eps, b = sigma, e0 = alpha
 struct PairP {
     float a, b, c, d;
     float cut2, ucut, scut, cut;
+    float e0, e1;
+    int kind, pad;
 };
 
 struct DP {
@@ -90,6 +96,7 @@ struct DP {
     int excl_cols;
     const GMEM float *raw_params;   // ntypes*ntypes*3
     int pair_base, cutoff;
+    float pexp0, pexp1;        // exponents fixed by the pair function (IPL n; LJ_m_n m, n; hertzian alpha)
     // integrator
     float dt, omega2, alpha;
     double degrees;
@@ -159,9 +166,61 @@ __device__ __forceinline__ float dist_sq_mic(float ax, float ay, float az, float
 // (LJ_12_6_sigma_epsilon.py:25-36, LJ_12_6.py:34-41), used for the cutoff
 // constants u(rc), s(rc) so they equal the oracle's bit for bit.
 // ---------------------------------------------------------------------------
-__device__ inline void pair_ref(int base, float dist, float p0, float p1, float &u, float &s, float &umm) {
+// r^-n for n > 0: by multiplication when n is a whole number up to 32, else
+// exp2(-n log2 r)
+__device__ __forceinline__ float inv_pow(float inv_r, float n) {
+    const int k = (int)n;
+    if ((float)k == n && k >= 0 && k <= 32) {
+        float p = 1.0f, x = inv_r;
+        for (int e = k; e > 0; e >>= 1) {
+            if (e & 1) p *= x;
+            x *= x;
+        }
+        return p;
+    }
+    return exp2f(n * log2f(inv_r));
+}
+
+// x^a for x >= 0, a > 0 (hertzian): exp2(a log2 x), 0 at x = 0
+__device__ __forceinline__ float pos_pow(float x, float a) {
+    return x > 0.0f ? exp2f(a * log2f(x)) : 0.0f;
+}
+
+// The pair functions of the family kind 2..5 at distance dist
+// (make_IPL_n.py:26-34, make_LJ_m_n.py:24-35, harmonic_repulsion.py:22-33,
+// hertzian.py:24-37); e0, e1 = the function's exponents.
+__device__ inline void pair_family(int kind, float dist, float p0, float p1, float e0, float e1, float &u, float &s,
+                                   float &umm) {
+    const float inv = 1.0f / dist;
+    if (kind == 2) {            // u = A r^-n
+        const float pn = inv_pow(inv, e0), pn2 = pn * inv * inv;
+        u = p0 * pn;
+        s = e0 * p0 * pn2;
+        umm = (e0 * (e0 + 1.0f)) * p0 * pn2;
+    } else if (kind == 3) {     // u = A_m r^-m + A_n r^-n
+        const float pm = inv_pow(inv, e0), pn = inv_pow(inv, e1), i2 = inv * inv;
+        u = p0 * pm + p1 * pn;
+        s = e0 * p0 * pm * i2 + e1 * p1 * pn * i2;
+        umm = (e0 * (e0 + 1.0f)) * p0 * pm * i2 + (e1 * (e1 + 1.0f)) * p1 * pn * i2;
+    } else if (kind == 4) {     // u = eps (1 - r/sigma)^2 / 2
+        const float is = 1.0f / p1, delta = 1.0f - dist * is;
+        u = 0.5f * p0 * delta * delta;
+        s = p0 * delta * is / dist;
+        umm = p0 * is * is;
+    } else {                    // u = eps (1 - r/sigma)^alpha
+        const float is = 1.0f / p1, delta = 1.0f - dist * is;
+        u = p0 * pos_pow(delta, e0);
+        s = p0 * e0 * pos_pow(delta, e0 - 1.0f) * is / dist;
+        umm = p0 * e0 * (e0 - 1.0f) * pos_pow(delta, e0 - 2.0f) * is * is;
+    }
+}
+
+__device__ inline void pair_ref(int base, float dist, float p0, float p1, float &u, float &s, float &umm,
+                                float e0 = 0.f, float e1 = 0.f) {
 #pragma clang fp contract(off)
-    if (base == 0) {
+    if (base >= 2) {
+        pair_family(base, dist, p0, p1, e0, e1, u, s, umm);
+    } else if (base == 0) {
         float od = 1.0f / dist;
         float x = p0 * od;
         float x2 = x * x;
@@ -184,8 +243,28 @@ __device__ inline void pair_ref(int base, float dist, float p0, float p1, float 
     }
 }
 
-__device__ inline void make_pairp(int base, int cutoff, float p0, float p1, float cut, PairP &q) {
-    if (base == 0) {
+__device__ inline void make_pairp(int base, int cutoff, float p0, float p1, float cut, PairP &q, float e0 = 0.f,
+                                  float e1 = 0.f) {
+    q.kind = base;
+    q.e0 = e0;
+    q.e1 = e1;
+    q.pad = 0;
+    if (base == 2) {
+        q.a = p0;
+        q.b = 0.0f;
+        q.c = e0 * p0;
+        q.d = 0.0f;
+    } else if (base == 3) {
+        q.a = p0;
+        q.b = p1;
+        q.c = e0 * p0;
+        q.d = e1 * p1;
+    } else if (base >= 4) {
+        q.a = p0;
+        q.b = p1;
+        q.c = 0.0f;
+        q.d = 0.0f;
+    } else if (base == 0) {
         q.a = p0 * p0;
         q.b = 4.0f * p1;
         q.c = 24.0f * p1;
@@ -202,7 +281,7 @@ __device__ inline void make_pairp(int base, int cutoff, float p0, float p1, floa
         q.cut2 = cut * cut;
     }
     float u, s, umm;
-    pair_ref(base, cut, p0, p1, u, s, umm);
+    pair_ref(base, cut, p0, p1, u, s, umm, e0, e1);
     q.ucut = (cutoff == 0) ? 0.0f : u;
     q.scut = (cutoff == 2) ? s : 0.0f;
 }
@@ -212,7 +291,13 @@ template <int BASE, int CUT>
 __device__ __forceinline__ void pair_fast(float d2, const PairP &q, float &u, float &s, float &umm, bool lap) {
     float ir2 = __builtin_amdgcn_rcpf(d2);
     umm = 0.f;
-    if (BASE == 0) {
+    if (BASE == 2) {
+        // the family IPL_n / LJ_m_n / harmonic / hertzian: a uniform switch on
+        // the kind (one per PairPotential), evaluated from the distance
+        const float dist = d2 * __builtin_amdgcn_rsqf(d2);
+        pair_family(q.kind, dist, q.a, q.b, q.e0, q.e1, u, s, umm);
+        if (!lap) umm = 0.f;
+    } else if (BASE == 0) {
         float x2 = q.a * ir2;
         float x6 = x2 * x2 * x2;
         float x12 = x6 * x6;

@@ -76,7 +76,7 @@ __device__ inline void load_pair_table(const DP &P, PairP *pp) {
     int nt2 = P.ntypes * P.ntypes;
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) {
         const GMEM float *q = P.raw_params + 3 * t;
-        make_pairp(P.pair_base, P.cutoff, q[0], q[1], q[2], pp[t]);
+        make_pairp(P.pair_base, P.cutoff, q[0], q[1], q[2], pp[t], P.pexp0, P.pexp1);
     }
 }
 

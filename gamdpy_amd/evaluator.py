@@ -33,6 +33,7 @@ class Evaluator:
         fields = {
             "N": conf.N, "D": conf.D, "ntypes": table.shape[0], "box": conf.simbox.lengths,
             "pair_base": _native.PAIR_BASE[ip["pair_base"]], "cutoff": _native.CUTOFF[ip["cutoff"]],
+            "pair_exps": ip.get("pair_exps", (0.0, 0.0)),
             "max_cut": float(ip["max_cut"]), "nblist_kind": _native.NBLIST[ip["nblist_kind"]],
             "skin": float(ip["skin"]), "max_nbs": int(ip["max_nbs"]),
             "utilize_niii": int(bool(self.compute_plan.get('UtilizeNIII', False))),

@@ -115,7 +115,7 @@ class PairPotential(Interaction):
         self.params_user = params
         self.exclusions = exclusions
         self.max_num_nbs = max_num_nbs
-        self.base, self.cutoff = pair_function_descriptor(pairpotential_function)
+        self.base, self.cutoff, self.exps = pair_function_descriptor(pairpotential_function)
         self.nblist = None
 
     def convert_user_params(self):
@@ -139,13 +139,30 @@ class PairPotential(Interaction):
         return params, max_cut
 
     def param_table(self):
-        """(ntypes, ntypes, 3) float32 table (p0, p1, cut) as the kernels read it."""
+        """(ntypes, ntypes, 3) float32 table (p0, p1, cut) as the kernels read
+        it, and the pair function's exponents: LJ_12_6_sigma_epsilon (sigma,
+        epsilon), LJ_12_6 (A12, A6), IPL_n (A_n, -), LJ_m_n (A_m, A_n),
+        harmonic_repulsion (eps, sigma), hertzian (eps, sigma; alpha, the same
+        for every type pair, goes with the exponents)."""
         params, max_cut = self.convert_user_params()
         nt = params.shape[0]
         names = params.dtype.names
         table = np.zeros((nt, nt, 3), np.float32)
         table[:, :, 0] = params[names[0]]
-        table[:, :, 1] = params[names[1]] if len(names) > 2 else 0.0
+        if self.base == "hertzian":
+            if len(names) < 3:
+                raise ValueError("hertzian takes the parameters eps, alpha, sigma[, cut]")
+            alpha = np.asarray(params[names[1]], np.float32)
+            if not np.all(alpha == alpha.flat[0]):
+                raise ValueError("hertzian: the MI355X kernels take one alpha for all type pairs")
+            table[:, :, 1] = params[names[2]]
+            self.exps = (float(alpha.flat[0]), 0.0)
+        elif self.base == "harmonic_repulsion":
+            table[:, :, 1] = params[names[1]]
+        elif self.base == "IPL_n":
+            table[:, :, 1] = 0.0
+        else:
+            table[:, :, 1] = params[names[1]] if len(names) > 2 else 0.0
         table[:, :, 2] = params[names[-1]]
         return table, max_cut
 
@@ -193,7 +210,8 @@ class PairPotential(Interaction):
         else:
             raise ValueError(f"No lblist called: {compute_plan['nblist']}. Use either 'N squared' or 'linked lists'")
         nblist_params = self.nblist.get_params(max_cut, compute_plan, verbose)
-        return {"pair_base": self.base, "cutoff": self.cutoff, "table": table, **nblist_params}
+        return {"pair_base": self.base, "cutoff": self.cutoff, "pair_exps": self.exps, "table": table,
+                **nblist_params}
 
     def get_kernel(self, configuration, compute_plan, compute_flags, verbose=False):
         """The device code is fixed HIP; this returns its descriptor."""

@@ -61,11 +61,100 @@ LJ_12_6_sigma_epsilon._gmd_base = "LJ_12_6_sigma_epsilon"
 LJ_12_6_sigma_epsilon._gmd_cutoff = None
 
 
+def _inv_pow(inv, n):
+    """inv^n in float32: by multiplication for whole n up to 32 (as the HIP
+    kernels evaluate it), else through exp2/log2."""
+    inv = _f32(inv)
+    k = int(n)
+    if float(k) == float(n) and 0 <= k <= 32:
+        p = np.ones_like(inv)
+        x = inv.copy()
+        e = k
+        while e > 0:
+            if e & 1:
+                p = (p * x).astype(F32)
+            x = (x * x).astype(F32)
+            e >>= 1
+        return p
+    return np.exp2(F32(n) * np.log2(inv)).astype(F32)
+
+
+def make_IPL_n(n, first_parameter=0):
+    """Inverse power law u(r) = A_n r^-n (make_IPL_n.py:6-36); params: A_n[, cut].
+    first_parameter picks the parameter that holds A_n (the kernels support 0)."""
+    nf = F32(n)
+
+    def IPL_n(dist, params):
+        an = _f32(params[first_parameter])
+        inv = F32(1.0) / _f32(dist)
+        pn = _inv_pow(inv, n)
+        pn2 = pn * inv * inv
+        return _out(an * pn, nf * an * pn2, F32(n * (n + 1)) * an * pn2)
+
+    IPL_n._gmd_base = "IPL_n" if first_parameter == 0 else None
+    IPL_n._gmd_cutoff = None
+    IPL_n._gmd_exps = (float(n), 0.0)
+    IPL_n.__name__ = f"IPL_{n}"
+    return IPL_n
+
+
+def make_LJ_m_n(m, n):
+    """Mie potential u(r) = A_m r^-m + A_n r^-n (make_LJ_m_n.py:6-37); params: A_m, A_n[, cut]."""
+
+    def LJ_m_n(dist, params):
+        am, an = _f32(params[0]), _f32(params[1])
+        inv = F32(1.0) / _f32(dist)
+        pm, pn = _inv_pow(inv, m), _inv_pow(inv, n)
+        i2 = inv * inv
+        u = am * pm + an * pn
+        s = F32(m) * am * pm * i2 + F32(n) * an * pn * i2
+        umm = F32(m * (m + 1)) * am * pm * i2 + F32(n * (n + 1)) * an * pn * i2
+        return _out(u, s, umm)
+
+    LJ_m_n._gmd_base = "LJ_m_n"
+    LJ_m_n._gmd_cutoff = None
+    LJ_m_n._gmd_exps = (float(m), float(n))
+    LJ_m_n.__name__ = f"LJ_{m}_{n}"
+    return LJ_m_n
+
+
+def harmonic_repulsion(dist, params):
+    """u(r) = eps (1 - r/sigma)^2 / 2 for r < sigma (harmonic_repulsion.py:4-35); params: eps, sigma[, cut]."""
+    eps, sigma = _f32(params[0]), _f32(params[1])
+    inv_sigma = F32(1.0) / sigma
+    delta = F32(1.0) - _f32(dist) * inv_sigma
+    u = F32(0.5) * eps * delta * delta
+    s = eps * delta * inv_sigma / _f32(dist)
+    umm = eps * inv_sigma * inv_sigma
+    return _out(u, s, umm * np.ones_like(u))
+
+
+harmonic_repulsion._gmd_base = "harmonic_repulsion"
+harmonic_repulsion._gmd_cutoff = None
+
+
+def hertzian(dist, params):
+    """u(r) = eps (1 - r/sigma)^alpha for r < sigma (hertzian.py:4-39); params: eps, alpha, sigma[, cut]."""
+    eps, alpha, sigma = _f32(params[0]), _f32(params[1]), _f32(params[2])
+    inv_sigma = F32(1.0) / sigma
+    delta = F32(1.0) - _f32(dist) * inv_sigma
+    one = F32(1.0)
+    u = eps * delta ** alpha
+    s = eps * alpha * delta ** (alpha - one) * inv_sigma / _f32(dist)
+    umm = eps * alpha * (alpha - one) * delta ** (alpha - F32(2.0)) * inv_sigma * inv_sigma
+    return _out(u, s, umm)
+
+
+hertzian._gmd_base = "hertzian"
+hertzian._gmd_cutoff = None
+
+
 def _base_of(pair_potential):
     base = getattr(pair_potential, "_gmd_base", None)
     if base is None or getattr(pair_potential, "_gmd_cutoff", None) is not None:
         raise ValueError(f"{getattr(pair_potential, '__name__', pair_potential)!r} is not a base pair function "
-                         "supported by the MI355X kernels (LJ_12_6, LJ_12_6_sigma_epsilon)")
+                         "supported by the MI355X kernels (LJ_12_6, LJ_12_6_sigma_epsilon, make_IPL_n, "
+                         "make_LJ_m_n, harmonic_repulsion, hertzian)")
     return base
 
 
@@ -81,6 +170,7 @@ def apply_shifted_potential_cutoff(pair_potential):
 
     potential._gmd_base = base
     potential._gmd_cutoff = "SP"
+    potential._gmd_exps = getattr(pair_potential, "_gmd_exps", (0.0, 0.0))
     potential.__name__ = f"shifted_potential({base})"
     return potential
 
@@ -98,15 +188,17 @@ def apply_shifted_force_cutoff(pair_potential):
 
     potential._gmd_base = base
     potential._gmd_cutoff = "SF"
+    potential._gmd_exps = getattr(pair_potential, "_gmd_exps", (0.0, 0.0))
     potential.__name__ = f"shifted_force({base})"
     return potential
 
 
 def pair_function_descriptor(func):
-    """(base, cutoff) of a supported pair function, else ValueError."""
+    """(base, cutoff, exponents) of a supported pair function, else ValueError."""
     base = getattr(func, "_gmd_base", None)
     if base is None:
         raise ValueError(f"pair function {getattr(func, '__name__', func)!r} has no HIP implementation; supported: "
-                         "LJ_12_6, LJ_12_6_sigma_epsilon, optionally wrapped by apply_shifted_potential_cutoff "
-                         "or apply_shifted_force_cutoff")
-    return base, getattr(func, "_gmd_cutoff", None)
+                         "LJ_12_6, LJ_12_6_sigma_epsilon, make_IPL_n(n), make_LJ_m_n(m, n), harmonic_repulsion, "
+                         "hertzian, optionally wrapped by apply_shifted_potential_cutoff or "
+                         "apply_shifted_force_cutoff")
+    return base, getattr(func, "_gmd_cutoff", None), tuple(getattr(func, "_gmd_exps", (0.0, 0.0)))

@@ -20,15 +20,17 @@
 extern "C" {
 #endif
 
-#define GMD_ABI_VERSION 5
+#define GMD_ABI_VERSION 6
 
 typedef struct gmd_ctx gmd_ctx;
 typedef struct gmd_rdf gmd_rdf;
 typedef struct gmd_batch gmd_batch;
 typedef struct gmd_adj gmd_adj;
 
-/* pair function (interactions/potential_functions/) */
-enum { GMD_LJ_12_6_SIGMA_EPSILON = 0, GMD_LJ_12_6 = 1 };
+/* pair function (interactions/potential_functions/): LJ_12_6_sigma_epsilon.py,
+ * LJ_12_6.py, make_IPL_n.py, make_LJ_m_n.py, harmonic_repulsion.py, hertzian.py */
+enum { GMD_LJ_12_6_SIGMA_EPSILON = 0, GMD_LJ_12_6 = 1, GMD_IPL_N = 2, GMD_LJ_M_N = 3, GMD_HARMONIC_REPULSION = 4,
+       GMD_HERTZIAN = 5 };
 /* cutoff wrapper: none, apply_shifted_potential_cutoff, apply_shifted_force_cutoff */
 enum { GMD_CUT_PLAIN = 0, GMD_CUT_SHIFTED_POTENTIAL = 1, GMD_CUT_SHIFTED_FORCE = 2 };
 /* compute_plan['nblist'] */
@@ -53,7 +55,12 @@ typedef struct gmd_desc {
 
     /* PairPotential (interactions/pair_potential.py:12-51) */
     int pair_base, cutoff;
-    const float *params;      /* ntypes*ntypes*3 float32: (p0, p1, cut) for [ti][tj] */
+    const float *params;      /* ntypes*ntypes*3 float32: (p0, p1, cut) for [ti][tj]:
+                                 LJ_12_6_sigma_epsilon (sigma, epsilon), LJ_12_6 (A12, A6),
+                                 IPL_n (A_n, -), LJ_m_n (A_m, A_n),
+                                 harmonic_repulsion (epsilon, sigma), hertzian (epsilon, sigma) */
+    float pair_exps[2];       /* exponents fixed by the pair function: IPL_n (n, -), LJ_m_n (m, n),
+                                 hertzian (alpha, -) (one alpha for all type pairs) */
     float max_cut;            /* max over the cut matrix (pair_potential.py:49) */
 
     /* NbList2 / NbListLinkedLists (interactions/nblist.py, nblist_linked_lists.py) */

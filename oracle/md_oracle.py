@@ -164,23 +164,74 @@ def lj_12_6(dist, a12, a6):
     return u, s, umm
 
 
-BASE_FUNCTIONS = {"LJ_12_6_sigma_epsilon": lj_sigma_epsilon, "LJ_12_6": lj_12_6}
+def ipl_n(dist, a, _unused, exps):
+    """make_IPL_n.py:26-34 in f32: u = A r^-n, s = n A r^-(n+2), u'' = n(n+1) A r^-(n+2)."""
+    n = exps[0]
+    inv = F32(1.0) / np.asarray(dist, F32)
+    a = np.asarray(a, F32)
+    u = a * inv ** F32(n)
+    s = F32(n) * a * inv ** F32(n + 2)
+    umm = F32(n * (n + 1)) * a * inv ** F32(n + 2)
+    return u, s, umm
 
 
-def pair_eval(base, cutoff, dist, p0, p1, cut):
+def lj_m_n(dist, am, an, exps):
+    """make_LJ_m_n.py:24-35 in f32: u = A_m r^-m + A_n r^-n."""
+    m, n = exps
+    inv = F32(1.0) / np.asarray(dist, F32)
+    am = np.asarray(am, F32)
+    an = np.asarray(an, F32)
+    u = am * inv ** F32(m) + an * inv ** F32(n)
+    s = F32(m) * am * inv ** F32(m + 2) + F32(n) * an * inv ** F32(n + 2)
+    umm = F32(m * (m + 1)) * am * inv ** F32(m + 2) + F32(n * (n + 1)) * an * inv ** F32(n + 2)
+    return u, s, umm
+
+
+def harmonic_repulsion(dist, eps, sigma, _exps=None):
+    """harmonic_repulsion.py:22-33 in f32: u = eps (1 - r/sigma)^2 / 2."""
+    dist = np.asarray(dist, F32)
+    eps = np.asarray(eps, F32)
+    inv_sigma = F32(1.0) / np.asarray(sigma, F32)
+    delta = F32(1.0) - dist * inv_sigma
+    u = F32(0.5) * eps * delta * delta
+    s = eps * delta * inv_sigma / dist
+    umm = eps * inv_sigma * inv_sigma * np.ones_like(u)
+    return u, s, umm
+
+
+def hertzian(dist, eps, sigma, exps):
+    """hertzian.py:24-37 in f32 with alpha = exps[0]: u = eps (1 - r/sigma)^alpha."""
+    alpha = F32(exps[0])
+    dist = np.asarray(dist, F32)
+    eps = np.asarray(eps, F32)
+    inv_sigma = F32(1.0) / np.asarray(sigma, F32)
+    delta = F32(1.0) - dist * inv_sigma
+    u = eps * delta ** alpha
+    s = eps * alpha * delta ** (alpha - F32(1.0)) * inv_sigma / dist
+    umm = eps * alpha * (alpha - F32(1.0)) * delta ** (alpha - F32(2.0)) * inv_sigma * inv_sigma
+    return u, s, umm
+
+
+BASE_FUNCTIONS = {"LJ_12_6_sigma_epsilon": lambda d, a, b, e: lj_sigma_epsilon(d, a, b),
+                  "LJ_12_6": lambda d, a, b, e: lj_12_6(d, a, b),
+                  "IPL_n": ipl_n, "LJ_m_n": lj_m_n, "harmonic_repulsion": harmonic_repulsion, "hertzian": hertzian}
+
+
+def pair_eval(base, cutoff, dist, p0, p1, cut, exps=(0.0, 0.0)):
     """Pair function with the cutoff wrapper applied (f32).
 
     cutoff 'SP': apply_shifted_potential_cutoff.py:41-47, u -= u(rc).
     cutoff 'SF': apply_shifted_force_cutoff.py:29-37,
                  u -= u(rc) - s(rc)*rc*(r - rc);  s -= s(rc).
     cutoff None: the bare function.
+    exps: the function's exponents (IPL_n: n; LJ_m_n: m, n; hertzian: alpha).
     """
     f = BASE_FUNCTIONS[base]
-    u, s, umm = f(dist, p0, p1)
+    u, s, umm = f(dist, p0, p1, exps)
     if cutoff is None:
         return u, s, umm
     cut = np.asarray(cut, F32)
-    u_c, s_c, _ = f(cut, p0, p1)
+    u_c, s_c, _ = f(cut, p0, p1, exps)
     if cutoff == "SP":
         return u - u_c, s, umm
     if cutoff == "SF":
@@ -296,7 +347,8 @@ def nblist_to_array(rows, max_nbs):
 # ----------------------------------------------------------------------------
 # Forces: interactions/pair_potential.py:141-246 (plain full-list calculator)
 # ----------------------------------------------------------------------------
-def pair_forces(r, ptype, box, params, rows, base="LJ_12_6_sigma_epsilon", cutoff="SP", niii=False):
+def pair_forces(r, ptype, box, params, rows, base="LJ_12_6_sigma_epsilon", cutoff="SP", niii=False,
+                exps=(0.0, 0.0)):
     """calc_forces (pair_potential.py:185-246) with the full-list calculator
     (:162-178) or the NIII calculator (:141-160), accumulated in list order.
 
@@ -334,7 +386,7 @@ def pair_forces(r, ptype, box, params, rows, base="LJ_12_6_sigma_epsilon", cutof
         jj = j[inside]
         drr = dr[inside]
         dist = np.sqrt(d2[inside])
-        u, s, umm = pair_eval(base, cutoff, dist, pp[inside, 0], pp[inside, 1], cut[inside])
+        u, s, umm = pair_eval(base, cutoff, dist, pp[inside, 0], pp[inside, 1], cut[inside], exps)
         for k in range(D):
             fk = drr[:, k] * s
             f[a, k] = f[a, k] - fk
@@ -350,7 +402,7 @@ def pair_forces(r, ptype, box, params, rows, base="LJ_12_6_sigma_epsilon", cutof
     return f, U, W, lap
 
 
-def pair_forces_f64(r, ptype, box, params, base="LJ_12_6_sigma_epsilon", cutoff="SP"):
+def pair_forces_f64(r, ptype, box, params, base="LJ_12_6_sigma_epsilon", cutoff="SP", exps=(0.0, 0.0)):
     """Brute-force all-pairs float64 evaluation (no list): the exact answer the
     float32 path approximates.  Returns f, U, W (per particle, full-list split)."""
     r = np.asarray(r, F64)
@@ -376,6 +428,19 @@ def pair_forces_f64(r, ptype, box, params, base="LJ_12_6_sigma_epsilon", cutoff=
             if base == "LJ_12_6_sigma_epsilon":
                 sr6 = (p0 / x) ** 6
                 return 4 * p1 * (sr6 * sr6 - sr6), 24 * p1 * (2 * sr6 * sr6 - sr6) / x ** 2
+            if base == "IPL_n":
+                n = exps[0]
+                return p0 * x ** -n, n * p0 * x ** -(n + 2)
+            if base == "LJ_m_n":
+                m, n = exps
+                return p0 * x ** -m + p1 * x ** -n, m * p0 * x ** -(m + 2) + n * p1 * x ** -(n + 2)
+            if base == "harmonic_repulsion":
+                dl = 1 - x / p1
+                return 0.5 * p0 * dl * dl, p0 * dl / p1 / x
+            if base == "hertzian":
+                a = exps[0]
+                dl = 1 - x / p1
+                return p0 * dl ** a, p0 * a * dl ** (a - 1) / p1 / x
             ir6 = x ** -6.0
             return p0 * ir6 * ir6 + p1 * ir6, (12 * p0 * ir6 * ir6 + 6 * p1 * ir6) / x ** 2
 
