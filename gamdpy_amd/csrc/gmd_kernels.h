@@ -90,15 +90,23 @@ __device__ __forceinline__ void wave_sync_lds() {
 // Cell grid: nblist_linked_lists.py:125-131, cell index evaluated in float64
 // (numba promotes r + 0.5*L to float64): floor((r + L/2) n / L) mod n.
 // ---------------------------------------------------------------------------
+// floor(...) mod n with Python's sign rule; a wrapped position gives a value
+// in [0, n], so the modulo is one compare in the common case
+__device__ __forceinline__ int cell_mod(double f, int n) {
+    int c = (int)f;
+    if ((unsigned)c >= (unsigned)n) {
+        c %= n;
+        if (c < 0) c += n;
+    }
+    return c;
+}
+
 __device__ __forceinline__ int cell_of(const DP &P, float x, float y, float z) {
     const double L0 = P.box0, L1 = P.box1, L2 = P.box2;
-    long long cx = (long long)floor(((double)x + 0.5 * L0) * (double)P.ncx / L0);
-    long long cy = (long long)floor(((double)y + 0.5 * L1) * (double)P.ncy / L1);
-    long long cz = (long long)floor(((double)z + 0.5 * L2) * (double)P.ncz / L2);
-    cx %= P.ncx; if (cx < 0) cx += P.ncx;
-    cy %= P.ncy; if (cy < 0) cy += P.ncy;
-    cz %= P.ncz; if (cz < 0) cz += P.ncz;
-    return (int)((cx * P.ncy + cy) * P.ncz + cz);
+    const int cx = cell_mod(floor(((double)x + 0.5 * L0) * (double)P.ncx / L0), P.ncx);
+    const int cy = cell_mod(floor(((double)y + 0.5 * L1) * (double)P.ncy / L1), P.ncy);
+    const int cz = cell_mod(floor(((double)z + 0.5 * L2) * (double)P.ncz / L2), P.ncz);
+    return (cx * P.ncy + cy) * P.ncz + cz;
 }
 
 __device__ __forceinline__ void cell_insert(const DP &P, int buf, int i, float x, float y, float z) {
@@ -478,11 +486,17 @@ __device__ __forceinline__ void lds_sort_segment(int *a, int b, int e) {
     }
 }
 
+// stencil cell sc (0..124) of cell (cx, cy, cz): offsets in [-2, 2] and at
+// least 5 cells per dimension, so each coordinate wraps at most once
 __device__ __forceinline__ int stencil_cell(const DP &P, int cx, int cy, int cz, int sc) {
-    int ix = sc / 25 - 2, iy = (sc / 5) % 5 - 2, iz = sc % 5 - 2;
-    int ox = (cx + ix) % P.ncx; if (ox < 0) ox += P.ncx;
-    int oy = (cy + iy) % P.ncy; if (oy < 0) oy += P.ncy;
-    int oz = (cz + iz) % P.ncz; if (oz < 0) oz += P.ncz;
+    const int ix = sc / 25 - 2, iy = (sc / 5) % 5 - 2, iz = sc % 5 - 2;
+    int ox = cx + ix, oy = cy + iy, oz = cz + iz;
+    ox += ox < 0 ? P.ncx : 0;
+    ox -= ox >= P.ncx ? P.ncx : 0;
+    oy += oy < 0 ? P.ncy : 0;
+    oy -= oy >= P.ncy ? P.ncy : 0;
+    oz += oz < 0 ? P.ncz : 0;
+    oz -= oz >= P.ncz ? P.ncz : 0;
     return (ox * P.ncy + oy) * P.ncz + oz;
 }
 
