@@ -534,16 +534,30 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
     const GMEM int *ccount = P.ccount + (size_t)I.cbuf * P.ncells;
     const GMEM int *cmem = P.cmem + (size_t)I.cbuf * P.ncells * P.ccap;
-    for (int c = B.bid * wpb + wid; c < P.ncells; c += B.nb * wpb) {
+    const int sb = lane + 64;
+    // counts of all 125 stencil cells (lane owns stencil cells lane and
+    // lane + 64) and the unsorted row of the cell itself (stencil cell 62);
+    // the next cell's are requested before this cell is worked on
+    struct CellHead {
+        int nca, ncb, na, nb, ownm;
+    };
+    auto head = [&](int c, CellHead &h) {
         const int cz = c % P.ncz, cy = (c / P.ncz) % P.ncy, cx = c / (P.ncz * P.ncy);
-        // counts of all 125 stencil cells (lane owns stencil cells lane and
-        // lane + 64) and the unsorted row of the cell itself (stencil cell 62)
-        const int sb = lane + 64;
-        const int nca = stencil_cell(P, cx, cy, cz, lane);
-        const int ncb = (sb < 125) ? stencil_cell(P, cx, cy, cz, sb) : 0;
-        const int na = min(ccount[nca], P.ccap);
-        const int nb = (sb < 125) ? min(ccount[ncb], P.ccap) : 0;
-        const int ownm = (lane < P.ccap) ? cmem[(size_t)c * P.ccap + lane] : 0;
+        h.nca = stencil_cell(P, cx, cy, cz, lane);
+        h.ncb = (sb < 125) ? stencil_cell(P, cx, cy, cz, sb) : 0;
+        h.na = ccount[h.nca];
+        h.nb = (sb < 125) ? ccount[h.ncb] : 0;
+        h.ownm = (lane < P.ccap) ? cmem[(size_t)c * P.ccap + lane] : 0;
+    };
+    const int cstride = B.nb * wpb;
+    CellHead nxt;
+    if (B.bid * wpb + wid < P.ncells) head(B.bid * wpb + wid, nxt);
+    for (int c = B.bid * wpb + wid; c < P.ncells; c += cstride) {
+        const CellHead cur = nxt;
+        if (c + cstride < P.ncells) head(c + cstride, nxt);
+        const int nca = cur.nca, ncb = cur.ncb, ownm = cur.ownm;
+        const int na = min(cur.na, P.ccap);
+        const int nb = (sb < 125) ? min(cur.nb, P.ccap) : 0;
         const int np = __shfl(na, 62, 64);
         if (np == 0) continue;
         // the cell's own particles in index order: lane q takes the q-th
