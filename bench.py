@@ -39,7 +39,7 @@ PROFILE_STEPS = 200
 # HBM bytes per k_step launch of this workload: rocprofv3 FETCH_SIZE + WRITE_SIZE
 # over the profiled launches, each scaled by its reading for a copy kernel of
 # known size (profiles/k_step_ka65536_pmc.md)
-TRAFFIC_BYTES = 5.389e7
+TRAFFIC_BYTES = 5.423e7
 
 
 def log(*a):
@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-replica and adjoint measurements")
     ap.add_argument("--replicas", type=int, default=128, help="replicas of the batched measurement")
     ap.add_argument("--replica-steps", type=int, default=1000, help="steps of the batched measurement")
+    ap.add_argument("--peaks", action="store_true",
+                    help="also measure the device's copy bandwidth and FP32 FMA rate (reported as device_peaks)")
     ap.add_argument("--large", action="store_true",
                     help="instead of the headline: single-component LJ, N=1048576 (FCC 64^3, rho 0.973), "
                          "NVT_Langevin T=0.7 alpha=0.1, linked lists skin 0.5, hipGraph path")
@@ -219,6 +221,11 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extras)
+    if args.peaks:
+        copy_ms = _native.copy_probe(1 << 30, reps=5)
+        out["device_peaks"] = {"copy_GBps": 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9,
+                               "fp32_fma_TFLOPs": _native.fma_probe(4096)}
+        log(f"[bench] device peaks: {out['device_peaks']}")
     print(json.dumps(out), flush=True)
 
 

@@ -100,6 +100,7 @@ _SIGNATURES = {
     "gmd_force_rebuild": (ctypes.c_int, [ctypes.c_void_p]),
     "gmd_set_reorder_interval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gmd_copy_probe": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, c_float_p]),
+    "gmd_fma_probe": (ctypes.c_int, [ctypes.c_int, c_float_p]),
     "gmd_get_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "gmd_set_thermostat": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "gmd_read_scalar_output": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
@@ -199,6 +200,13 @@ def copy_probe(nbytes, reps=3):
     ms = ctypes.c_float()
     check(load().gmd_copy_probe(int(nbytes), int(reps), ctypes.byref(ms)), "gmd_copy_probe")
     return float(ms.value)
+
+
+def fma_probe(iters=4096):
+    """Achieved FP32 TFLOP/s of independent FMA chains (no packed FP32)."""
+    tf = ctypes.c_float()
+    check(load().gmd_fma_probe(int(iters), ctypes.byref(tf)), "gmd_fma_probe")
+    return float(tf.value)
 
 
 def fill_desc(d, desc_fields, params, exclusions=None):

@@ -422,6 +422,24 @@ __global__ void __launch_bounds__(256) k_rdf_boxes(const float4 *pos_p, int N, i
     }
 }
 
+// Independent FMA chains (8 per lane) for gmd_fma_probe: 16 flop per lane and
+// iteration, the sums stored so nothing is optimised away.
+__global__ void __launch_bounds__(256) k_fma(int iters, float *out_p) {
+    GMEM float *out = (GMEM float *)out_p;
+    float a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = (float)(threadIdx.x + k) * 1e-3f;
+    const float m = 0.999f, c = 1e-4f;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = fmaf(a[k], m, c);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += a[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 // Plain streaming copy (n float4), the known-size HBM workload of gmd_copy_probe.
 __global__ void __launch_bounds__(256) k_copy(const float4 *src_p, float4 *dst_p, size_t n) {
     const GMEM float4 *src = (const GMEM float4 *)src_p;
@@ -1442,6 +1460,38 @@ int gmd_copy_probe(unsigned long long bytes, int reps, float *ms_per_copy) {
     if (st) hipStreamDestroy(st);
     if (src) hipFree(src);
     if (dst) hipFree(dst);
+    return rc;
+}
+
+int gmd_fma_probe(int iters, float *tflops) {
+    if (!tflops || iters < 1) return fail("gmd_fma_probe: bad argument");
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = cus * 8, block = 256;
+    float *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&out, (size_t)grid * block * sizeof(float)) != hipSuccess) {
+        rc = fail("gmd_fma_probe: out of device memory");
+    } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+               hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail("gmd_fma_probe: stream or event creation failed");
+    } else {
+        hipLaunchKernelGGL(k_fma, dim3(grid), dim3(block), 0, st, iters, out);   // warm-up
+        bool ok = hipEventRecord(e0, st) == hipSuccess;
+        hipLaunchKernelGGL(k_fma, dim3(grid), dim3(block), 0, st, iters, out);
+        ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, st) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess;
+        float ms = 0.f;
+        if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail("gmd_fma_probe: launch failed");
+        else *tflops = (float)(16.0 * iters * (double)grid * block / (ms * 1e-3) / 1e12);
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (out) hipFree(out);
     return rc;
 }
 

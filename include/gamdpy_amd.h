@@ -156,6 +156,11 @@ int gmd_set_reorder_interval(gmd_ctx *ctx, int steps);
  * byte counters are scaled (a measurement aid; no counterpart in the
  * reference). */
 int gmd_copy_probe(unsigned long long bytes, int reps, float *ms_per_copy);
+/* Vector FP32 probe: independent fmaf chains (8 per lane, 'iters' each) over
+ * 8 blocks of 256 threads per compute unit; the achieved FP32 TFLOP/s of the
+ * code this library is compiled to (no packed FP32).  A measurement aid; no
+ * counterpart in the reference. */
+int gmd_fma_probe(int iters, float *tflops);
 
 /* NVT.thermostat_state (NVT.py:143): [xi, sum K] */
 int gmd_get_thermostat(gmd_ctx *ctx, float state[2]);
