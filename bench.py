@@ -84,6 +84,11 @@ def main():
     ap.add_argument("--replica-steps", type=int, default=1000, help="steps of the batched measurement")
     ap.add_argument("--peaks", action="store_true",
                     help="also measure the device's copy bandwidth and FP32 FMA rate (reported as device_peaks)")
+    ap.add_argument("--small", action="store_true",
+                    help="instead of the headline: LJ N=2048 (FCC 8^3, rho 0.8442, T 1.44), NVE dt 0.005, "
+                         "N-squared list skin 0.5, --steps timed steps (the reference's LJ benchmark size)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="instead of the headline: only the batched-replica measurement (--replicas, --replica-steps)")
     ap.add_argument("--large", action="store_true",
                     help="instead of the headline: single-component LJ, N=1048576 (FCC 64^3, rho 0.973), "
                          "NVT_Langevin T=0.7 alpha=0.1, linked lists skin 0.5, hipGraph path")
@@ -105,6 +110,12 @@ def main():
         copy_calibration(args.copy_calibration)
     if args.large:
         print(json.dumps(large_system(args)), flush=True)
+        return
+    if args.small:
+        print(json.dumps(small_system(args)), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batched_replicas": batched_speedup(args.replicas, args.replica_steps)}), flush=True)
         return
 
     conf, pp = make_system()
@@ -227,6 +238,47 @@ def main():
                                "fp32_fma_TFLOPs": _native.fma_probe(4096)}
         log(f"[bench] device peaks: {out['device_peaks']}")
     print(json.dumps(out), flush=True)
+
+
+def small_system(args):
+    """LJ (SP cutoff 2.5), FCC 8^3 at rho 0.8442 (N = 2048), T = 1.44, NVE
+    dt = 0.005 with MomentumReset(100), N-squared list skin 0.5: the
+    reference's LJ benchmark size (benchmarks/benchmark_LJ.py).  --driver
+    picks graph (default) or gridsync; --steps timed steps after --warmup."""
+    import gamdpy_amd as gp
+    conf = gp.Configuration(D=3)
+    conf.make_lattice(gp.unit_cells.FCC, cells=[8, 8, 8], rho=0.8442)
+    conf['m'] = 1.0
+    conf.randomize_velocities(temperature=1.44, seed=0)
+    N = conf.N
+    pp = gp.PairPotential(gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon), params=[1.0, 1.0, 2.5],
+                          max_num_nbs=256)
+    plan = gp.get_default_compute_plan(conf)
+    plan['skin'] = 0.5
+    plan['nblist'] = 'N squared'
+    if args.tp:
+        plan['tp'] = args.tp
+        plan['pb'] = max(1, 256 // args.tp)
+    if args.driver:
+        plan['gridsync'] = args.driver == "gridsync"
+        plan['graph'] = args.driver == "graph"
+    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.005), [gp.MomentumReset(100)], num_timeblocks=1,
+                        steps_per_timeblock=args.steps, compute_plan=dict(plan), storage='memory')
+    sim.integrate_self(0.0, args.warmup)
+    ctx = sim.ctx
+    nb0 = ctx.nbflag()[2]
+    sim.integrate_self(np.float32(args.warmup * 0.005), args.steps)
+    gpu_ms = ctx.last_run_ms()
+    tps = args.steps / (gpu_ms * 1e-3)
+    log(f"[bench] small: N={N} plan={plan}: {tps:.0f} TPS, {N * tps:.3e} particle-steps/s, "
+        f"{gpu_ms * 1e3 / args.steps:.2f} us/step, rebuilds {int(ctx.nbflag()[2] - nb0)}")
+    return {"metric": "TPS LJ N=2048 NVE", "value": tps, "unit": "steps/s", "particle_steps_per_s": N * tps,
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": gpu_ms / args.steps,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "LJ SP cutoff 2.5, FCC 8^3 rho=0.8442 T=1.44, NVE dt=0.005, N-squared list "
+                                   "skin 0.5", "N": N,
+                       "plan": {k: plan[k] for k in ('pb', 'tp', 'gridsync', 'nblist')} | {"graph": plan.get('graph', True)}},
+            "nblist_rebuilds_timed": int(ctx.nbflag()[2] - nb0)}
 
 
 def large_system(args):
