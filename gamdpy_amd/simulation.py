@@ -265,6 +265,8 @@ class Simulation:
                 interaction.check_datastructure_validity()
             for runtime_action in self.runtime_actions:
                 runtime_action.update_at_end_of_timeblock(block, self.output, ctx=self.ctx)
+            if hasattr(self.memory, "flush"):
+                self.memory.flush()
             yield block
         if self.timing:
             self.timing_numba = (time.perf_counter() - t_start) * 1000.0
