@@ -9,14 +9,17 @@ import sys
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
-SOURCES = [os.path.join(_HERE, "csrc", "gmd_core.hip")]
-DEPS = [os.path.join(_HERE, "csrc", f) for f in ("gmd_device.h", "gmd_kernels.h", "gmd_adjoint.h")] + \
+# gmd_core.hip holds the host code, the base-0 step kernels and every other
+# kernel; each further pair-function base has its own instance file, so the
+# three compile in parallel and are linked into one library
+SOURCES = [os.path.join(_HERE, "csrc", f) for f in ("gmd_core.hip", "gmd_steps_b1.hip", "gmd_steps_b2.hip")]
+DEPS = [os.path.join(_HERE, "csrc", f) for f in ("gmd_device.h", "gmd_kernels.h", "gmd_steps.h", "gmd_adjoint.h")] + \
        [os.path.join(ROOT, "include", "gamdpy_amd.h")]
 OUT = os.path.join(_HERE, "libgamdpy_amd.so")
 ARCH = os.environ.get("GMD_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Wno-unused-result",
     # no packed FP32 instructions are generated (v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -31,15 +34,36 @@ def hipcc():
 
 
 def build(verbose=False):
-    """Compile every HIP source into gamdpy_amd/libgamdpy_amd.so (always recompiles)."""
+    """Compile every HIP source (in parallel) and link gamdpy_amd/libgamdpy_amd.so
+    (always recompiles)."""
+    cc = hipcc()
+    inc = ["-I", os.path.join(ROOT, "include")]
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = OUT + "." + os.path.splitext(os.path.basename(src))[0] + ".o"
+        cmd = [cc] + HIPCC_FLAGS + inc + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True), src))
+        objs.append(obj)
+    failed = []
+    for proc, src in procs:
+        out, _ = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError(f"hipcc failed for {', '.join(os.path.basename(f) for f in failed)}")
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include")] + SOURCES + ["-o", tmp]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
+    for obj in objs:
+        os.remove(obj)
     os.replace(tmp, OUT)
     return OUT
 

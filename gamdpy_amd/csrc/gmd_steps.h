@@ -1,0 +1,133 @@
+// gmd_steps.h -- the step kernels as templates over (pair function, cutoff,
+// integrator): included by gmd_core.hip (which instantiates the LJ
+// sigma/epsilon family and declares the others extern) and by one file per
+// further pair-function base, so the instances compile in parallel.
+#pragma once
+#include "gmd_kernels.h"
+
+// ============================================================================
+// Drivers
+// ============================================================================
+// Every step of a run inside one launch; the blocks of one system meet at a
+// software barrier after each step, and the last to arrive runs the tail.
+// a pointer that is the same in every lane, made visibly uniform (loads
+// through it become scalar loads)
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
+template <int BASE, int CUT, int INTEG>
+__device__ __forceinline__ void persistent_body(const DP *Pp, const BlockCtx &B) {
+    extern __shared__ __align__(16) char lds[];
+    PairP *pp = (PairP *)(lds + Pp->lds_pp);
+    double *red = (double *)(lds + Pp->lds_red);
+    char *build = lds + Pp->lds_build;
+    load_pair_table(*Pp, pp);
+    __syncthreads();
+    const int nsteps = Pp->ctrl->nsteps;
+    for (int s = 0; s <= nsteps; ++s) {
+        const DP &P = *Pp;
+        const bool rebuild = ld_flag(P.rebuild_flag + s) != 0;
+        const StepInfo I = step_info(P, s, INTEG, rebuild);
+        float acc[GMD_NRED];
+#pragma unroll
+        for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
+        step_body<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
+        const unsigned mask = red_mask(I.need_ksum, I.need_mom, I.need_save);
+        if (mask) block_partials(P, B, acc, red, P.partial, mask);
+        grid_barrier(P.ctrl, B.nb, [&]() { step_tail(P, I, red, B.nb); });
+    }
+}
+
+// One step.  FOLD (one system): the step is s_base + k, the rebuild decision
+// is the flag the previous step raised, and the previous step's tail runs in
+// every block here (folded_tail); the partial sums alternate between two
+// buffers.  Otherwise (replicas of a batch) the step index and the rebuild
+// decision come from the control block that the previous step's k_batch_tail
+// wrote.
+template <int BASE, int CUT, int INTEG, bool FOLD>
+__device__ __forceinline__ void step_once(const DP &P, const BlockCtx &B, int k) {
+    extern __shared__ __align__(16) char lds[];
+    const int s = FOLD ? P.ctrl->s_base + k : P.ctrl->step;
+    if (s > P.ctrl->nsteps) return;
+    const bool rebuild = FOLD ? ld_flag(P.rebuild_flag + s) != 0 : P.ctrl->rebuild_now != 0;
+    PairP *pp = (PairP *)(lds + P.lds_pp);
+    double *red = (double *)(lds + P.lds_red);
+    char *build = lds + P.lds_build;
+    StepInfo I = step_info(P, s, INTEG, rebuild);
+    float pv[RED_PRE][GMD_NRED];
+    const unsigned qmask = FOLD ? folded_tail_load<INTEG>(P, I, pv) : 0u;
+    load_pair_table(P, pp);
+    if (FOLD) folded_tail<INTEG>(P, B, I, red, qmask, pv);
+    __syncthreads();
+    float acc[GMD_NRED];
+#pragma unroll
+    for (int c = 0; c < GMD_NRED; ++c) acc[c] = 0.f;
+    step_body<BASE, CUT, INTEG>(P, B, I, pp, build, acc);
+    const unsigned mask = red_mask(I.need_ksum, I.need_mom, I.need_save);
+    if (mask) block_partials(P, B, acc, red, FOLD ? P.partial + (size_t)(s & 1) * GMD_NRED * P.pstride : P.partial, mask);
+    if (FOLD && rebuild && B.bid == 0 && threadIdx.x == 0) P.ctrl->nbflag[2] += 1;
+}
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_persistent(const DP *__restrict__ Pp) {
+    persistent_body<BASE, CUT, INTEG>(Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x});
+}
+
+// One step per launch (step s_base + k).  The cross-block work of a step
+// (sums, thermostat, momentum shift, saver row) runs at the start of the
+// next step's launch in every block (folded_tail): no one-block tail launch
+// between steps, and no last-block-done scheme, which would need an
+// agent-scope release per block (a write-back of the XCD's L2 under the
+// blocks still running).
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256, 4) k_step(const DP *__restrict__ Pp, int k) {
+    step_once<BASE, CUT, INTEG, true>(*Pp, BlockCtx{(int)blockIdx.x, (int)gridDim.x}, k);
+}
+
+// Batched replicas: block b of the grid works for replica blk[b].rep as its
+// block blk[b].bid of blk[b].nb.  Replicas never wait for each other: the
+// persistent form synchronises only the blocks of one replica.
+struct BatchBlk {
+    int rep, bid, nb, pad;
+};
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_batch_persistent(const DP *const *Ps, const BatchBlk *blks) {
+    const BatchBlk a = blks[blockIdx.x];
+    const int rep = __builtin_amdgcn_readfirstlane(a.rep);
+    if (rep < 0) return;
+    const BlockCtx B = {__builtin_amdgcn_readfirstlane(a.bid), __builtin_amdgcn_readfirstlane(a.nb)};
+    persistent_body<BASE, CUT, INTEG>(uniform_ptr(Ps[rep]), B);
+}
+
+template <int BASE, int CUT, int INTEG>
+__global__ void __launch_bounds__(256) k_batch_step(const DP *const *Ps, const BatchBlk *blks) {
+    const BatchBlk a = blks[blockIdx.x];
+    const int rep = __builtin_amdgcn_readfirstlane(a.rep);
+    if (rep < 0) return;
+    const BlockCtx B = {__builtin_amdgcn_readfirstlane(a.bid), __builtin_amdgcn_readfirstlane(a.nb)};
+    step_once<BASE, CUT, INTEG, false>(*uniform_ptr(Ps[rep]), B, 0);
+}
+
+
+// explicit instantiation (EXT empty) or declaration (EXT extern) of the four
+// step kernels of one (base, cutoff, integrator)
+#define GMD_STEP_KERNELS(EXT, B, C, I)                                                          \
+    EXT template __global__ void k_persistent<B, C, I>(const DP *);                              \
+    EXT template __global__ void k_step<B, C, I>(const DP *, int);                               \
+    EXT template __global__ void k_batch_persistent<B, C, I>(const DP *const *, const BatchBlk *); \
+    EXT template __global__ void k_batch_step<B, C, I>(const DP *const *, const BatchBlk *);
+#define GMD_STEP_KERNELS_CUT(EXT, B, C) \
+    GMD_STEP_KERNELS(EXT, B, C, 0)      \
+    GMD_STEP_KERNELS(EXT, B, C, 1)      \
+    GMD_STEP_KERNELS(EXT, B, C, 2)      \
+    GMD_STEP_KERNELS(EXT, B, C, 3)
+#define GMD_STEP_KERNELS_BASE(EXT, B) \
+    GMD_STEP_KERNELS_CUT(EXT, B, 0)   \
+    GMD_STEP_KERNELS_CUT(EXT, B, 1)   \
+    GMD_STEP_KERNELS_CUT(EXT, B, 2)
