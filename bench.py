@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-replica and adjoint measurements")
     ap.add_argument("--replicas", type=int, default=128, help="replicas of the batched measurement")
     ap.add_argument("--replica-steps", type=int, default=1000, help="steps of the batched measurement")
+    ap.add_argument("--rdf-sync", action="store_true",
+                    help="compute each RDF update before the next steps start (default: overlapped with them)")
     ap.add_argument("--peaks", action="store_true",
                     help="also measure the device's copy bandwidth and FP32 FMA rate (reported as device_peaks)")
     ap.add_argument("--small", action="store_true",
@@ -169,14 +171,25 @@ def main():
     rdf_count = 0
     done = 0
     t0 = time.perf_counter()
+    pending = False
     while done < steps:
         k = min(RDF_EVERY, steps - done)
         sim.integrate_self(np.float32(done * DT), k)
         kernel_ms.append((ctx.last_run_ms(), k))
         done += k
         if k == RDF_EVERY:
-            rdf_h.compute_ctx(ctx)      # synchronous: returns the counts on the host
+            if args.rdf_sync:
+                rdf_h.compute_ctx(ctx)      # synchronous: returns the counts on the host
+            else:
+                # snapshot now, histogram on the calculator's stream while the
+                # next block's steps run; counts collected before the next one
+                if pending:
+                    rdf_h.collect()
+                rdf_h.launch_ctx(ctx)
+                pending = True
             rdf_count += 1
+    if pending:
+        rdf_h.collect()                 # inside the timed region
     elapsed = time.perf_counter() - t0
     rebuilds = int(ctx.nbflag()[2] - nb0)
     # per-kernel durations with hipEvents around every launch, on the

@@ -107,6 +107,8 @@ _SIGNATURES = {
     "gmd_rdf_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "gmd_rdf_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "gmd_rdf_compute_ctx": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int_p, c_u64_p]),
+    "gmd_rdf_launch_ctx": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_int_p]),
+    "gmd_rdf_collect": (ctypes.c_int, [ctypes.c_void_p, c_u64_p]),
     "gmd_rdf_compute_host": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_int_p, c_float_p, c_u64_p]),
     "gmd_batch_create": (ctypes.c_int, [ctypes.POINTER(GmdDesc), ctypes.c_int, c_int_p, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
@@ -423,6 +425,18 @@ class RdfHandle:
         out = np.zeros((self.ntypes, self.ntypes, self.bins), np.uint64)
         pt = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.int32)
         check(self.lib.gmd_rdf_compute_ctx(self.h, ctx.h, iptr(pt), u64ptr(out)), "gmd_rdf_compute_ctx")
+        return out
+
+    def launch_ctx(self, ctx, ptype=None):
+        """Start an update from the context's current positions; returns at once."""
+        pt = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.int32)
+        self._pt = pt   # kept alive until the call returns (the library copies it)
+        check(self.lib.gmd_rdf_launch_ctx(self.h, ctx.h, iptr(pt)), "gmd_rdf_launch_ctx")
+
+    def collect(self):
+        """Counts of the launched update (waits for it)."""
+        out = np.zeros((self.ntypes, self.ntypes, self.bins), np.uint64)
+        check(self.lib.gmd_rdf_collect(self.h, u64ptr(out)), "gmd_rdf_collect")
         return out
 
     def compute_host(self, r, ptype, box):

@@ -30,7 +30,8 @@ class CalculatorRadialDistribution:
         self.bins = bins
         self.count = 0
         self.compute_plan = compute_plan
-        self.rdf_list = []
+        self._rdf_list = []
+        self._pending = False
         nptypes = int(np.max(ptype if ptype is not None else configuration.ptype)) + 1
         self.gr_bins = np.zeros((nptypes, nptypes, self.bins), dtype=np.float64)
         self._h = _native.RdfHandle(configuration.N, nptypes, bins)
@@ -38,16 +39,37 @@ class CalculatorRadialDistribution:
     def _ptype(self):
         return self.ptype_user if self.ptype_user is not None else self.configuration.ptype
 
-    def update(self):
-        """Add the current configuration's pair counts (calculator_radial_distribution.py:111-119)."""
+    @property
+    def rdf_list(self):
+        """Counts of every update so far (an overlapped update is waited for here)."""
+        self._collect()
+        return self._rdf_list
+
+    def _collect(self):
+        if self._pending:
+            self._pending = False
+            self._rdf_list.append(self._h.collect().astype(np.float64))
+
+    def update(self, overlap=False):
+        """Add the current configuration's pair counts (calculator_radial_distribution.py:111-119).
+
+        overlap=True (configuration on the device): snapshot the positions and
+        histogram them on the calculator's own stream while the simulation's
+        next steps run; the counts are the same and are collected at the next
+        update or when rdf_list / read() is used."""
         self.count += 1
         conf = self.configuration
         ctx = getattr(conf, "_ctx", None)
+        self._collect()
         if ctx is not None:
+            if overlap:
+                self._h.launch_ctx(ctx, self.ptype_user)
+                self._pending = True
+                return
             counts = self._h.compute_ctx(ctx, self.ptype_user)
         else:
             counts = self._h.compute_host(conf['r'], self._ptype(), conf.simbox.lengths)
-        self.rdf_list.append(counts.astype(np.float64))
+        self._rdf_list.append(counts.astype(np.float64))
 
     def read(self):
         """Normalised total and partial g(r) (calculator_radial_distribution.py:121-159)."""

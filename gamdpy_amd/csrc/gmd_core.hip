@@ -623,6 +623,9 @@ struct gmd_rdf {
     float box[3] = {0, 0, 0};
     std::vector<float> thr;
     hipStream_t stream = nullptr;
+    hipEvent_t snap = nullptr, done = nullptr;   // snapshot taken on the context's stream; result on the host
+    unsigned long long *h_out = nullptr;         // pinned: the reduced counts of the last launch
+    bool pending = false;                        // a launched update not collected yet
 };
 
 extern "C" {
@@ -1497,9 +1500,16 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
         gmd_rdf_destroy(r);
         return fail("gmd_rdf_create: out of device memory");
     }
-    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&r->snap, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess) {
         gmd_rdf_destroy(r);
-        return fail("gmd_rdf_create: stream creation failed");
+        return fail("gmd_rdf_create: stream or event creation failed");
+    }
+    if (hipHostMalloc((void **)&r->h_out, nh * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        r->h_out = nullptr;
+        gmd_rdf_destroy(r);
+        return fail("gmd_rdf_create: out of pinned host memory");
     }
     *out = r;
     return 0;
@@ -1510,12 +1520,18 @@ int gmd_rdf_destroy(gmd_rdf *r) {
     void *ptrs[] = {r->pos, r->tbox, r->ptype, r->d_thr, r->part, r->d_out};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (r->pending && r->done) hipEventSynchronize(r->done);
     if (r->stream) hipStreamDestroy(r->stream);
+    if (r->snap) hipEventDestroy(r->snap);
+    if (r->done) hipEventDestroy(r->done);
+    if (r->h_out) hipHostFree(r->h_out);
     delete r;
     return 0;
 }
 
-static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3], unsigned long long *counts) {
+// Enqueue one histogram of the positions at pos on the calculator's stream,
+// ending in an asynchronous copy of the counts to pinned host memory.
+static int rdf_launch(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3]) {
     if (box[0] != r->box[0] || box[1] != r->box[1] || box[2] != r->box[2] || r->thr.empty()) {
         rdf_thresholds(box, r->bins, r->thr);
         HIPCHECK(hipMemcpy(r->d_thr, r->thr.data(), (size_t)(r->bins + 1) * sizeof(float), hipMemcpyHostToDevice));
@@ -1552,9 +1568,18 @@ static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const floa
     hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256), (unsigned)((r->grid + per - 1) / per)),
                        dim3(256), 0, r->stream, (const unsigned *)r->part, r->grid, (int)nh, per, r->d_out);
     HIPCHECK(hipGetLastError());
-    std::vector<unsigned long long> h(nh);
-    HIPCHECK(hipMemcpyAsync(h.data(), r->d_out, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));
-    HIPCHECK(hipStreamSynchronize(r->stream));
+    HIPCHECK(hipMemcpyAsync(r->h_out, r->d_out, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));
+    HIPCHECK(hipEventRecord(r->done, r->stream));
+    r->pending = true;
+    return 0;
+}
+
+// Wait for the launched histogram and fold it into ordered-pair counts.
+static int rdf_collect(gmd_rdf *r, unsigned long long *counts) {
+    if (!r->pending) return fail("gmd_rdf_collect: no update is pending");
+    HIPCHECK(hipEventSynchronize(r->done));
+    r->pending = false;
+    const unsigned long long *h = r->h_out;
     // unordered pair counts in ordered-type slots -> ordered pair counts
     const int nt = r->nt, bins = r->bins;
     for (int a = 0; a < nt; ++a)
@@ -1567,14 +1592,14 @@ static int rdf_run(gmd_rdf *r, const float4 *pos, const int *d_ptype, const floa
     return 0;
 }
 
-int gmd_rdf_compute_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, unsigned long long *counts) {
-    if (!r || !c || !counts) return fail("gmd_rdf_compute_ctx: null argument");
-    if (c->d.N != r->N) return fail("gmd_rdf_compute_ctx: N mismatch (%d vs %d)", c->d.N, r->N);
-    HIPCHECK(hipStreamSynchronize(c->stream));
+int gmd_rdf_launch_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype) {
+    if (!r || !c) return fail("gmd_rdf_launch_ctx: null argument");
+    if (c->d.N != r->N) return fail("gmd_rdf_launch_ctx: N mismatch (%d vs %d)", c->d.N, r->N);
+    if (r->pending) return fail("gmd_rdf_launch_ctx: the previous update is not collected");
     const int *dp = nullptr;
     if (ptype) {
         for (int i = 0; i < r->N; ++i)
-            if (ptype[i] < 0 || ptype[i] >= r->nt) return fail("gmd_rdf_compute_ctx: type out of range");
+            if (ptype[i] < 0 || ptype[i] >= r->nt) return fail("gmd_rdf_launch_ctx: type out of range");
         std::vector<int> perm, pt(ptype, ptype + r->N);
         if (host_perm(c, perm)) return -1;
         if (!perm.empty())
@@ -1582,9 +1607,26 @@ int gmd_rdf_compute_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, unsigned long 
         HIPCHECK(hipMemcpy(r->ptype, pt.data(), (size_t)r->N * sizeof(int), hipMemcpyHostToDevice));
         dp = r->ptype;
     } else if (c->d.ntypes > r->nt) {
-        return fail("gmd_rdf_compute_ctx: context has %d types, calculator %d", c->d.ntypes, r->nt);
+        return fail("gmd_rdf_launch_ctx: context has %d types, calculator %d", c->d.ntypes, r->nt);
     }
-    return rdf_run(r, c->buf0 ? c->pos1 : c->pos0, dp, c->d.box, counts);
+    // snapshot of the current positions, ordered after the context's work so
+    // far; the context's later steps may overwrite its buffers meanwhile
+    HIPCHECK(hipMemcpyAsync(r->pos, c->buf0 ? c->pos1 : c->pos0, (size_t)r->N * sizeof(float4),
+                            hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipEventRecord(r->snap, c->stream));
+    HIPCHECK(hipStreamWaitEvent(r->stream, r->snap, 0));
+    return rdf_launch(r, r->pos, dp, c->d.box);
+}
+
+int gmd_rdf_collect(gmd_rdf *r, unsigned long long *counts) {
+    if (!r || !counts) return fail("gmd_rdf_collect: null argument");
+    return rdf_collect(r, counts);
+}
+
+int gmd_rdf_compute_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, unsigned long long *counts) {
+    if (!r || !c || !counts) return fail("gmd_rdf_compute_ctx: null argument");
+    if (gmd_rdf_launch_ctx(r, c, ptype)) return -1;
+    return rdf_collect(r, counts);
 }
 
 int gmd_rdf_compute_host(gmd_rdf *r, const float *pos, const int *ptype, const float box[3],
@@ -1594,8 +1636,10 @@ int gmd_rdf_compute_host(gmd_rdf *r, const float *pos, const int *ptype, const f
         if (ptype[i] < 0 || ptype[i] >= r->nt) return fail("gmd_rdf_compute_host: type out of range");
     std::vector<float4> h(r->N);
     for (int i = 0; i < r->N; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], i2f(ptype[i]));
+    if (r->pending) return fail("gmd_rdf_compute_host: a launched update is not collected");
     HIPCHECK(hipMemcpy(r->pos, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
-    return rdf_run(r, r->pos, nullptr, box, counts);
+    if (rdf_launch(r, r->pos, nullptr, box)) return -1;
+    return rdf_collect(r, counts);
 }
 
 

@@ -176,6 +176,14 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out);
 int gmd_rdf_destroy(gmd_rdf *rdf);
 /* positions from the context's current device state (no host copy) */
 int gmd_rdf_compute_ctx(gmd_rdf *rdf, gmd_ctx *ctx, const int *ptype, unsigned long long *counts);
+/* gmd_rdf_compute_ctx in two halves: launch snapshots the context's current
+ * positions (on the context's stream) and histograms them on the
+ * calculator's own stream, returning at once, so the context's next steps
+ * run meanwhile; collect waits for that histogram and returns its counts.
+ * One launch at a time per calculator.  The reference's update
+ * (calculator_radial_distribution.py:111-119) is the two back to back. */
+int gmd_rdf_launch_ctx(gmd_rdf *rdf, gmd_ctx *ctx, const int *ptype);
+int gmd_rdf_collect(gmd_rdf *rdf, unsigned long long *counts);
 /* positions from a host array (N,3) */
 int gmd_rdf_compute_host(gmd_rdf *rdf, const float *r, const int *ptype, const float box[3],
                          unsigned long long *counts);

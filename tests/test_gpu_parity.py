@@ -218,6 +218,25 @@ def test_rdf_from_simulation_context_matches_host():
     assert np.array_equal(calc.rdf_list[0].astype(np.int64), ref)
 
 
+def test_rdf_overlapped_updates_equal_their_snapshots():
+    """update(overlap=True): each histogram runs while the next timeblock's
+    steps run and must equal the oracle's counts of the positions at the
+    update (bit for bit)."""
+    conf = ka_fcc([6, 6, 6], T=1.0)
+    pp = ka_pair_potential(max_nbs=300)
+    sim = gp.Simulation(conf, pp, gp.NVE(dt=0.004), [], num_timeblocks=3, steps_per_timeblock=40,
+                        compute_plan=plan(tp=4), storage='memory')
+    calc = gp.CalculatorRadialDistribution(conf, bins=300)
+    refs = []
+    for _ in sim.run_timeblocks():
+        refs.append(orc.rdf_counts(conf['r'], conf.ptype, conf.simbox.lengths, 300))
+        calc.update(overlap=True)
+    assert len(calc.rdf_list) == 3
+    for got, ref in zip(calc.rdf_list, refs):
+        assert np.array_equal(got.astype(np.int64), ref)
+    assert not np.array_equal(refs[0], refs[2])   # the configuration moved between updates
+
+
 def test_rdf_ideal_gas():
     """Reference test_radial_distrubution.py:9-37 (D=3): mean g(r) = 1 +- 0.01."""
     for rho in (1.0, 0.32):
