@@ -619,7 +619,8 @@ struct gmd_rdf {
     float *d_thr = nullptr;
     unsigned *part = nullptr;
     unsigned long long *d_out = nullptr;
-    int grid = 0;
+    int grid = 0;      // histogram blocks of a launch whose result is awaited at once
+    int grid_bg = 0;   // fewer blocks for a launch that runs beside the simulation's steps
     float box[3] = {0, 0, 0};
     std::vector<float> thr;
     hipStream_t stream = nullptr;
@@ -1491,6 +1492,9 @@ int gmd_rdf_create(int N, int ntypes, int bins, gmd_rdf **out) {
     // a block's 32-bit counters see at most 2^31 pairs
     const long long need = (long long)(((double)N * (double)N / 2.0) / 2147483648.0) + 1;
     r->grid = (int)std::max(1LL, std::min(npairs, std::max((long long)cus * 4, need)));
+    // beside the step kernels, 2 blocks per CU take fewer of their slots and
+    // still finish within a block of steps
+    r->grid_bg = (int)std::max(1LL, std::min(npairs, std::max((long long)cus * 2, need)));
     if (hipMalloc(&r->pos, (size_t)N * sizeof(float4)) != hipSuccess ||
         hipMalloc(&r->tbox, (size_t)4 * r->nchunks * sizeof(float4)) != hipSuccess ||
         hipMalloc(&r->ptype, (size_t)N * sizeof(int)) != hipSuccess ||
@@ -1531,7 +1535,7 @@ int gmd_rdf_destroy(gmd_rdf *r) {
 
 // Enqueue one histogram of the positions at pos on the calculator's stream,
 // ending in an asynchronous copy of the counts to pinned host memory.
-static int rdf_launch(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3]) {
+static int rdf_launch(gmd_rdf *r, const float4 *pos, const int *d_ptype, const float box[3], int grid) {
     if (box[0] != r->box[0] || box[1] != r->box[1] || box[2] != r->box[2] || r->thr.empty()) {
         rdf_thresholds(box, r->bins, r->thr);
         HIPCHECK(hipMemcpy(r->d_thr, r->thr.data(), (size_t)(r->bins + 1) * sizeof(float), hipMemcpyHostToDevice));
@@ -1562,11 +1566,11 @@ static int rdf_launch(gmd_rdf *r, const float4 *pos, const int *d_ptype, const f
     HIPCHECK(hipMemsetAsync(r->d_out, 0, nh * sizeof(unsigned long long), r->stream));
     hipLaunchKernelGGL(k_rdf_boxes, dim3((unsigned)((r->nchunks + 3) / 4)), dim3(256), 0, r->stream, pos, r->N,
                        r->nchunks, box[0], box[1], box[2], r->tbox);
-    hipLaunchKernelGGL(k_rdf, dim3(r->grid), dim3(256), rdf_lds_bytes(r->nt, r->bins), r->stream, R);
+    hipLaunchKernelGGL(k_rdf, dim3(grid), dim3(256), rdf_lds_bytes(r->nt, r->bins), r->stream, R);
     HIPCHECK(hipGetLastError());
     const int per = 16;
-    hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256), (unsigned)((r->grid + per - 1) / per)),
-                       dim3(256), 0, r->stream, (const unsigned *)r->part, r->grid, (int)nh, per, r->d_out);
+    hipLaunchKernelGGL(k_rdf_reduce, dim3((unsigned)((nh + 255) / 256), (unsigned)((grid + per - 1) / per)),
+                       dim3(256), 0, r->stream, (const unsigned *)r->part, grid, (int)nh, per, r->d_out);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(r->h_out, r->d_out, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));
     HIPCHECK(hipEventRecord(r->done, r->stream));
@@ -1592,7 +1596,7 @@ static int rdf_collect(gmd_rdf *r, unsigned long long *counts) {
     return 0;
 }
 
-int gmd_rdf_launch_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype) {
+static int rdf_launch_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, int grid) {
     if (!r || !c) return fail("gmd_rdf_launch_ctx: null argument");
     if (c->d.N != r->N) return fail("gmd_rdf_launch_ctx: N mismatch (%d vs %d)", c->d.N, r->N);
     if (r->pending) return fail("gmd_rdf_launch_ctx: the previous update is not collected");
@@ -1615,7 +1619,12 @@ int gmd_rdf_launch_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype) {
                             hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(hipEventRecord(r->snap, c->stream));
     HIPCHECK(hipStreamWaitEvent(r->stream, r->snap, 0));
-    return rdf_launch(r, r->pos, dp, c->d.box);
+    return rdf_launch(r, r->pos, dp, c->d.box, grid);
+}
+
+int gmd_rdf_launch_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype) {
+    if (!r) return fail("gmd_rdf_launch_ctx: null argument");
+    return rdf_launch_ctx(r, c, ptype, r->grid_bg);
 }
 
 int gmd_rdf_collect(gmd_rdf *r, unsigned long long *counts) {
@@ -1625,7 +1634,7 @@ int gmd_rdf_collect(gmd_rdf *r, unsigned long long *counts) {
 
 int gmd_rdf_compute_ctx(gmd_rdf *r, gmd_ctx *c, const int *ptype, unsigned long long *counts) {
     if (!r || !c || !counts) return fail("gmd_rdf_compute_ctx: null argument");
-    if (gmd_rdf_launch_ctx(r, c, ptype)) return -1;
+    if (rdf_launch_ctx(r, c, ptype, r->grid)) return -1;
     return rdf_collect(r, counts);
 }
 
@@ -1638,7 +1647,7 @@ int gmd_rdf_compute_host(gmd_rdf *r, const float *pos, const int *ptype, const f
     for (int i = 0; i < r->N; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], i2f(ptype[i]));
     if (r->pending) return fail("gmd_rdf_compute_host: a launched update is not collected");
     HIPCHECK(hipMemcpy(r->pos, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
-    if (rdf_launch(r, r->pos, nullptr, box)) return -1;
+    if (rdf_launch(r, r->pos, nullptr, box, r->grid)) return -1;
     return rdf_collect(r, counts);
 }
 
