@@ -8,6 +8,10 @@ and against independent rumd3 data.
   configurational temperature, virial/energy correlation R and slope Gamma.
   These bands are the reference's; they also pin the Langevin integrator,
   whose noise stream cannot be compared bit for bit.
+* Langevin at a literature state point: the reference's
+  test_nvt_langevin.py:1-68 (LJ T = 1.2, rho = 1/0.9672, the fcc-liquid
+  coexistence point of doi:10.1063/1.4818747, total energy -4.020 per
+  particle): mean potential and kinetic energy per particle within 0.05.
 * KA at rho = 1.2, T = 2.0: mean potential energy and virial per particle
   against rumd3's energies_mean.dat of the same state point (N = 4000,
   shifted-potential cutoff 2.5 sigma_ij), starting from rumd3's own snapshot.
@@ -103,6 +107,32 @@ def test_lj_langevin_bands(cells):
     assert 0.62 < Tconf < 0.78
     assert 0.85 < R < 1.00
     assert 4.8 < Gamma < 6.8
+
+
+def test_langevin_literature_state_point():
+    """test_nvt_langevin.py:10-68: 7^3 FCC, SP cutoff 2.5, dt = 0.005,
+    alpha = 0.1, MomentumReset(100), ScalarSaver(32), 32 blocks of 512 steps,
+    means over blocks 1-31."""
+    temperature = 1.2
+    expected_k = 3 / 2 * temperature
+    expected_u = -4.020 - expected_k
+    configuration = gp.Configuration(D=3, compute_flags={'W': True, 'K': True})
+    configuration.make_lattice(gp.unit_cells.FCC, cells=[7, 7, 7], rho=1 / 0.9672)
+    configuration['m'] = 1.0
+    configuration.randomize_velocities(temperature=2 * temperature, seed=0)
+    pairfunc = gp.apply_shifted_potential_cutoff(gp.LJ_12_6_sigma_epsilon)
+    pairpot = gp.PairPotential(pairfunc, params=[1.0, 1.0, 2.5], max_num_nbs=1000)
+    integrator = gp.integrators.NVT_Langevin(temperature, alpha=0.1, dt=0.005, seed=0)
+    runtime_actions = [gp.MomentumReset(100), gp.ScalarSaver(32, {'W': True, 'K': True})]
+    sim = gp.Simulation(configuration, pairpot, integrator, runtime_actions, num_timeblocks=32,
+                        steps_per_timeblock=512, storage='memory')
+    for _ in sim.run_timeblocks():
+        pass
+    U, K = (np.asarray(x, np.float64) / configuration.N
+            for x in gp.extract_scalars(sim.output, ['U', 'K'], first_block=1))
+    print(U.mean(), expected_u, K.mean(), expected_k)
+    assert abs(U.mean() - expected_u) < 0.05
+    assert abs(K.mean() - expected_k) < 0.05
 
 
 def _ka_rumd3(T, steps_per_block, blocks, save_every, extra=(), nblist='N squared'):
