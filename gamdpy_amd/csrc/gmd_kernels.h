@@ -401,6 +401,7 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
+    const float iL0 = P.ibox0, iL1 = P.ibox1, iL2 = P.ibox2;
     const bool lap = P.compute_lap != 0;
     // the descriptor fields the filter needs, held in registers (the list
     // stores would otherwise make the compiler reload them every batch)
@@ -418,29 +419,25 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
         int cntq = __shfl(mycount, q, 64);
         int nacc = 0;
         for (int kk = 0; kk < ce; kk += 64) {
-            bool ok = false;
-            int j = -1;
-            if (kk + lane < ce) {
-                const float4 cj = cand[kk + lane];
-                j = __float_as_int(cj.w) & 0xffffff;
-                bool acc;
-                if (niii) {
-                    const int two_dn = 2 * (j - iq);   // nblist.py:112-116
-                    acc = (two_dn > 0 && two_dn <= N) || (two_dn < -N);
-                } else {
-                    acc = j != iq;
-                }
-                if (acc) {
-                    float dx, dy, dz;
-                    const float d2 = dist_sq_mic(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, dx, dy, dz);
-                    ok = d2 < cps2 && !(has_excl && excluded(P, iq, j));
-                }
+            // branch-free test of every lane (kk + lane < LL_IDXCAP, so the
+            // read stays inside the staging buffer; lanes past ce are masked)
+            const float4 cj = cand[kk + lane];
+            const int j = __float_as_int(cj.w) & 0xffffff;
+            const int two_dn = 2 * (j - iq);   // nblist.py:112-116
+            const bool acc = niii ? ((two_dn > 0 && two_dn <= N) || (two_dn < -N)) : (j != iq);
+            // list minimum image: exact for every pair closer than L/2, and
+            // the stencil needs L >= 2.5 (rc + skin), so membership is exact
+            float dx, dy, dz;
+            const float d2 = dist_sq_list(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, iL0, iL1, iL2, dx, dy, dz);
+            bool ok = (kk + lane < ce) & acc & (d2 < cps2);
+            if (has_excl) {
+                if (ok) ok = !excluded(P, iq, j);
             }
             const unsigned long long mask = __ballot(ok);
             if (ok) {
                 const int pre = __popcll(mask & lt_mask);
                 const int slot = cntq + pre;
-                if (slot < maxnb) nbq[(size_t)(slot >> tsh) * rs + (slot & (tp - 1))] = j;
+                if (slot < maxnb) nbq[(size_t)((unsigned)slot >> tsh) * rs + (unsigned)(slot & (tp - 1))] = j;
                 accs[nacc + pre] = (unsigned short)(kk + lane);
             }
             const int na = __popcll(mask);
@@ -452,7 +449,7 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
         for (int e = lane; e < nacc; e += 64) {
             const float4 cj = cand[accs[e]];
             float dx, dy, dz;
-            const float d2 = dist_sq_mic(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, dx, dy, dz);
+            const float d2 = dist_sq_list(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, iL0, iL1, iL2, dx, dy, dz);
             const PairP &qq = ppq[__float_as_int(cj.w) >> 24];
             if (d2 < qq.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, qq, fx, fy, fz, uu, ww, ll, lap);
         }
