@@ -387,6 +387,31 @@ __device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const
 // inside the pair cutoff contribute to the force at once.  Lane q finally
 // holds the totals of particle q and integrates it.
 // ---------------------------------------------------------------------------
+// Sums of six per-lane values over the wavefront.  The first three exchange
+// levels halve the set of values a lane carries (eight slots, the last two
+// zero): after them lane l holds a partial of slot l >> 3 over the eight
+// lanes that differ from it in bits 3-5, and three more levels complete it.
+// The total of slot c is then in lanes 8c .. 8c + 7 (10 shuffles instead of
+// 36 for six separate reductions).
+__device__ __forceinline__ float wave_sum6(float a0, float a1, float a2, float a3, float a4, float a5) {
+    const int lane = threadIdx.x & 63;
+    const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
+    // level 32: lanes below 32 keep slots 0-3, the others slots 4-7
+    const float y0 = (h5 ? a4 : a0) + __shfl_xor(h5 ? a0 : a4, 32, 64);
+    const float y1 = (h5 ? a5 : a1) + __shfl_xor(h5 ? a1 : a5, 32, 64);
+    const float y2 = (h5 ? 0.f : a2) + __shfl_xor(h5 ? a2 : 0.f, 32, 64);
+    const float y3 = (h5 ? 0.f : a3) + __shfl_xor(h5 ? a3 : 0.f, 32, 64);
+    // level 16: keep two of the four
+    const float z0 = (h4 ? y2 : y0) + __shfl_xor(h4 ? y0 : y2, 16, 64);
+    const float z1 = (h4 ? y3 : y1) + __shfl_xor(h4 ? y1 : y3, 16, 64);
+    // level 8: keep one
+    float w = (h3 ? z1 : z0) + __shfl_xor(h3 ? z0 : z1, 8, 64);
+    w += __shfl_xor(w, 4, 64);
+    w += __shfl_xor(w, 2, 64);
+    w += __shfl_xor(w, 1, 64);
+    return w;
+}
+
 // Test the staged candidates cand[0, ce) against each particle of the cell:
 // cand holds (x, y, z, index | type << 24) per candidate.  For particle q the
 // 64 lanes first filter the candidates (list test and list write, ballot
@@ -394,7 +419,7 @@ __device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const
 // evaluate q's pair terms over the accepted ones only, every lane busy,
 // instead of carrying the force through the filter where most lanes of a
 // 64-candidate batch hold rejected pairs.
-template <int BASE, int CUT>
+template <int BASE, int CUT, bool EXCL>
 __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const float4 *cand, unsigned short *accs,
                                           const int ce, const int np, const int myi, const float4 myr, int &mycount,
                                           float q6[6]) {
@@ -406,7 +431,7 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
     // the descriptor fields the filter needs, held in registers (the list
     // stores would otherwise make the compiler reload them every batch)
     const int N = P.N, tp = P.tp, tsh = P.tp_shift, maxnb = P.max_nbs;
-    const bool niii = P.niii != 0, has_excl = P.excl != nullptr;
+    const bool niii = P.niii != 0;
     const float cps2 = P.cps2;
     GMEM int *const nbl = P.nbl;
     const size_t rs = (size_t)N * (size_t)tp;   // list row stride
@@ -430,7 +455,7 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
             float dx, dy, dz;
             const float d2 = dist_sq_list(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, iL0, iL1, iL2, dx, dy, dz);
             bool ok = (kk + lane < ce) & acc & (d2 < cps2);
-            if (has_excl) {
+            if (EXCL) {
                 if (ok) ok = !excluded(P, iq, j);
             }
             const unsigned long long mask = __ballot(ok);
@@ -454,20 +479,14 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
             if (d2 < qq.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, qq, fx, fy, fz, uu, ww, ll, lap);
         }
         wave_sync_lds();   // accs is rewritten for the next particle
-        fx = wave_sum_f(fx);
-        fy = wave_sum_f(fy);
-        fz = wave_sum_f(fz);
-        uu = wave_sum_f(uu);
-        ww = wave_sum_f(ww);
-        if (lap) ll = wave_sum_f(ll);
+        const int w = __float_as_int(wave_sum6(fx, fy, fz, uu, ww, ll));
+        float t[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t[c] = __int_as_float(__builtin_amdgcn_readlane(w, 8 * c));
         if (lane == q) {
             mycount = cntq;
-            q6[0] += fx;
-            q6[1] += fy;
-            q6[2] += fz;
-            q6[3] += uu;
-            q6[4] += ww;
-            q6[5] += ll;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) q6[c] += t[c];
         }
     }
 }
@@ -618,7 +637,12 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
                 }
             }
             wave_sync_lds();
-            cell_test<BASE, CUT>(P, pp, cand, accs, ce, np, myi, myr, mycount, q6);
+            // exclusions in a variant of their own: the common case keeps its
+            // filter free of the exclusion-row scan
+            if (P.excl)
+                cell_test<BASE, CUT, true>(P, pp, cand, accs, ce, np, myi, myr, mycount, q6);
+            else
+                cell_test<BASE, CUT, false>(P, pp, cand, accs, ce, np, myi, myr, mycount, q6);
             done = pe;
         }
         if (myi >= 0) {
