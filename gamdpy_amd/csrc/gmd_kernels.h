@@ -387,6 +387,12 @@ __device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const
 // inside the pair cutoff contribute to the force at once.  Lane q finally
 // holds the totals of particle q and integrates it.
 // ---------------------------------------------------------------------------
+// The value of lane l (l wave-uniform) in every lane, via a scalar register.
+__device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // Sums of six per-lane values over the wavefront.  The first three exchange
 // levels halve the set of values a lane carries (eight slots, the last two
 // zero): after them lane l holds a partial of slot l >> 3 over the eight
@@ -436,12 +442,13 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
     GMEM int *const nbl = P.nbl;
     const size_t rs = (size_t)N * (size_t)tp;   // list row stride
     for (int q = 0; q < np; ++q) {
-        const int iq = __shfl(myi, q, 64);
-        const float rx = __shfl(myr.x, q, 64), ry = __shfl(myr.y, q, 64), rz = __shfl(myr.z, q, 64);
-        const int tq = __float_as_int(__shfl(myr.w, q, 64));
+        // q is wave-uniform: lane q's values are read into scalar registers
+        const int iq = lane_i(myi, q);
+        const float rx = lane_f(myr.x, q), ry = lane_f(myr.y, q), rz = lane_f(myr.z, q);
+        const int tq = lane_i(__float_as_int(myr.w), q);
         const PairP *ppq = pp + tq * P.ntypes;
         GMEM int *const nbq = nbl + (size_t)iq * tp;
-        int cntq = __shfl(mycount, q, 64);
+        const int cnt0 = lane_i(mycount, q);   // q's list entries before this pass
         int nacc = 0;
         for (int kk = 0; kk < ce; kk += 64) {
             // branch-free test of every lane (kk + lane < LL_IDXCAP, so the
@@ -459,20 +466,18 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
                 if (ok) ok = !excluded(P, iq, j);
             }
             const unsigned long long mask = __ballot(ok);
-            if (ok) {
-                const int pre = __popcll(mask & lt_mask);
-                const int slot = cntq + pre;
-                if (slot < maxnb) nbq[(size_t)((unsigned)slot >> tsh) * rs + (unsigned)(slot & (tp - 1))] = j;
-                accs[nacc + pre] = (unsigned short)(kk + lane);
-            }
-            const int na = __popcll(mask);
-            cntq += na;
-            nacc += na;
+            if (ok) accs[nacc + __popcll(mask & lt_mask)] = (unsigned short)(kk + lane);
+            nacc += __popcll(mask);
         }
         wave_sync_lds();
+        // the accepted candidates in candidate order: list slots cnt0 + e
+        // (written here, two passes of 64 instead of one store per batch)
         float fx = 0.f, fy = 0.f, fz = 0.f, uu = 0.f, ww = 0.f, ll = 0.f;
         for (int e = lane; e < nacc; e += 64) {
             const float4 cj = cand[accs[e]];
+            const int slot = cnt0 + e;
+            if (slot < maxnb)
+                nbq[(size_t)((unsigned)slot >> tsh) * rs + (unsigned)(slot & (tp - 1))] = __float_as_int(cj.w) & 0xffffff;
             float dx, dy, dz;
             const float d2 = dist_sq_list(cj.x, cj.y, cj.z, rx, ry, rz, L0, L1, L2, iL0, iL1, iL2, dx, dy, dz);
             const PairP &qq = ppq[__float_as_int(cj.w) >> 24];
@@ -484,7 +489,7 @@ __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const fl
 #pragma unroll
         for (int c = 0; c < 6; ++c) t[c] = __int_as_float(__builtin_amdgcn_readlane(w, 8 * c));
         if (lane == q) {
-            mycount = cntq;
+            mycount = cnt0 + nacc;
 #pragma unroll
             for (int c = 0; c < 6; ++c) q6[c] += t[c];
         }
@@ -574,11 +579,11 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
         const int nca = cur.nca, ncb = cur.ncb, ownm = cur.ownm;
         const int na = min(cur.na, P.ccap);
         const int nb = (sb < 125) ? min(cur.nb, P.ccap) : 0;
-        const int np = __shfl(na, 62, 64);
+        const int np = lane_i(na, 62);
         if (np == 0) continue;
         // the cell's own particles in index order: lane q takes the q-th
         int rank = 0;
-        for (int q = 0; q < np; ++q) rank += (__shfl(ownm, q, 64) < ownm) ? 1 : 0;
+        for (int q = 0; q < np; ++q) rank += (lane_i(ownm, q) < ownm) ? 1 : 0;
         wave_sync_lds();
         if (lane < np) own[rank] = ownm;
         wave_sync_lds();
@@ -598,9 +603,9 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
                 ib += tb;
             }
         }
-        const int totA = __shfl(ia, 63, 64);
+        const int totA = lane_i(ia, 63);
         ib += totA;
-        const int M = __shfl(ib, 63, 64);
+        const int M = lane_i(ib, 63);
         const int offa = ia - na, offb = ib - nb;
         int mycount = 0;
         float q6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
