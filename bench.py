@@ -39,7 +39,7 @@ PROFILE_STEPS = 200
 # HBM bytes per k_step launch of this workload: rocprofv3 FETCH_SIZE + WRITE_SIZE
 # over the profiled launches, each scaled by its reading for a copy kernel of
 # known size (profiles/k_step_ka65536_pmc.md)
-TRAFFIC_BYTES = 5.423e7
+TRAFFIC_BYTES = 5.350e7
 
 
 def log(*a):
