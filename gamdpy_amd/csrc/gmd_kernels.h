@@ -382,10 +382,10 @@ __device__ __forceinline__ void step_lists(const DP &P, const BlockCtx &B, const
 // Rebuild step, linked lists: one wavefront per cell.  The wavefront stages
 // the members of the 125 stencil cells (cell >= (rc+skin)/2,
 // nblist_linked_lists.py:151-182) in LDS, then for each particle of its cell
-// tests them 64 at a time: accepted neighbours are appended to the list with
-// a ballot (stencil order, then particle index: deterministic) and those
-// inside the pair cutoff contribute to the force at once.  Lane q finally
-// holds the totals of particle q and integrates it.
+// tests them 64 at a time: accepted neighbours are appended to the list in
+// ballot order (stencil order, then particle index: deterministic) and those
+// inside the pair cutoff contribute to the force in the same pass.  Lane q
+// finally holds the totals of particle q and integrates it.
 // ---------------------------------------------------------------------------
 // The value of lane l (l wave-uniform) in every lane, via a scalar register.
 __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
@@ -420,10 +420,10 @@ __device__ __forceinline__ float wave_sum6(float a0, float a1, float a2, float a
 
 // Test the staged candidates cand[0, ce) against each particle of the cell:
 // cand holds (x, y, z, index | type << 24) per candidate.  For particle q the
-// 64 lanes first filter the candidates (list test and list write, ballot
-// order), recording the accepted candidates' slots in accs; then they
-// evaluate q's pair terms over the accepted ones only, every lane busy,
-// instead of carrying the force through the filter where most lanes of a
+// 64 lanes first filter the candidates (the list test only, compacted in
+// ballot order into the slot list accs); then, over the accepted ones only,
+// every lane busy, they write q's list entries and evaluate its pair terms,
+// instead of carrying both through the filter where most lanes of a
 // 64-candidate batch hold rejected pairs.
 template <int BASE, int CUT, bool EXCL>
 __device__ __forceinline__ void cell_test(const DP &P, const PairP *pp, const float4 *cand, unsigned short *accs,
