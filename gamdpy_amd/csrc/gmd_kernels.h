@@ -679,6 +679,11 @@ __device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B,
     const float L0 = P.box0, L1 = P.box1, L2 = P.box2;
     cpos_t pos = I.cur ? P.pos1 : P.pos0;
     const bool lap = P.compute_lap != 0;
+    const int N = P.N, maxnb = P.max_nbs;
+    const bool niii = P.niii != 0, has_excl = P.excl != nullptr;
+    const float cps2 = P.cps2;
+    // this wave's accepted tile slots (a tile holds bd <= 256 positions)
+    unsigned short *accs = (unsigned short *)(tile + bd) + wid * 256;
     for (int rd = 0; rd < rounds; ++rd) {
         const int i = (rd * B.nb + B.bid) * wpb + wid;   // wave-uniform
         const bool valid = i < P.N;
@@ -695,26 +700,42 @@ __device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B,
             const int jl = t0 + (int)threadIdx.x;
             if (jl < P.N) tile[threadIdx.x] = pos[jl];
             __syncthreads();
-            const int tn = min(bd, P.N - t0);
+            const int tn = min(bd, N - t0);
             if (!valid) continue;
+            // the list test only, branch-free (e < bd: inside the tile; lanes
+            // past tn are masked), accepted slots compacted in ballot order
+            int nacc = 0;
             for (int c0 = 0; c0 < tn; c0 += 64) {
-                const int j = t0 + c0 + lane;
-                bool ok = false;
-                if (c0 + lane < tn && pair_accept(P, i, j)) {
-                    const float4 rj = tile[c0 + lane];
-                    float dx, dy, dz;
-                    const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
-                    ok = d2 < P.cps2 && !excluded(P, i, j);
-                    const PairP &q = ppi[__float_as_int(rj.w)];
-                    if (ok && d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
+                const int e = c0 + lane;
+                const float4 rj = tile[e];
+                const int j = t0 + e;
+                const int two_dn = 2 * (j - i);   // nblist.py:112-116
+                const bool acc = niii ? ((two_dn > 0 && two_dn <= N) || (two_dn < -N)) : (j != i);
+                float dx, dy, dz;
+                const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
+                bool ok = (e < tn) & acc & (d2 < cps2);
+                if (has_excl) {
+                    if (ok) ok = !excluded(P, i, j);
                 }
                 const unsigned long long mask = __ballot(ok);
-                if (ok) {
-                    const int slot = cnt + __popcll(mask & lt_mask);
-                    if (slot < P.max_nbs) P.nbl[nbl_index(P, i, slot)] = j;
-                }
-                cnt += __popcll(mask);
+                if (ok) accs[nacc + __popcll(mask & lt_mask)] = (unsigned short)e;
+                nacc += __popcll(mask);
             }
+            wave_sync_lds();
+            // the accepted pairs of the tile, every lane busy: list entries
+            // (slots cnt + a, tile order) and pair terms
+            for (int a = lane; a < nacc; a += 64) {
+                const int e = accs[a];
+                const float4 rj = tile[e];
+                const int slot = cnt + a;
+                if (slot < maxnb) P.nbl[nbl_index(P, i, slot)] = t0 + e;
+                float dx, dy, dz;
+                const float d2 = dist_sq_mic(rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, L0, L1, L2, dx, dy, dz);
+                const PairP &q = ppi[__float_as_int(rj.w)];
+                if (d2 < q.cut2) pair_accum<BASE, CUT>(d2, dx, dy, dz, q, fx, fy, fz, uu, ww, ll, lap);
+            }
+            cnt += nacc;
+            wave_sync_lds();   // accs is rewritten for the next tile
         }
         if (valid) {
             fx = wave_sum_f(fx);
