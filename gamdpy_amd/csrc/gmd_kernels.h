@@ -663,10 +663,11 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
 // ---------------------------------------------------------------------------
 // Rebuild step, N squared (nblist.py:89-143): one wavefront per particle i.
 // The block stages 256 positions at a time in LDS; each wavefront tests them
-// against its particle 64 at a time and appends the accepted j with a ballot,
-// so the list is in ascending j; pairs inside the cutoff contribute to the
-// force at once, and lane 0 integrates the particle.  Every wavefront of
-// every block runs the same number of rounds (the tiles are shared).
+// against its particle 64 at a time and compacts the accepted tile slots with
+// a ballot, then appends those j to the list (so it is in ascending j) and
+// adds their pair terms in one pass, every lane busy; lane 0 integrates the
+// particle.  Every wavefront of every block runs the same number of rounds
+// (the tiles are shared).
 // ---------------------------------------------------------------------------
 template <int BASE, int CUT, int INTEG>
 __device__ __forceinline__ void step_nsq_rebuild(const DP &P, const BlockCtx &B, const StepInfo &I, const PairP *pp,
