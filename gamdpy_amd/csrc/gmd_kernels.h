@@ -562,8 +562,7 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
     struct CellHead {
         int nca, ncb, na, nb, ownm;
     };
-    auto head = [&](int c, CellHead &h) {
-        const int cz = c % P.ncz, cy = (c / P.ncz) % P.ncy, cx = c / (P.ncz * P.ncy);
+    auto head = [&](int c, int cx, int cy, int cz, CellHead &h) {
         h.nca = stencil_cell(P, cx, cy, cz, lane);
         h.ncb = (sb < 125) ? stencil_cell(P, cx, cy, cz, sb) : 0;
         h.na = ccount[h.nca];
@@ -571,11 +570,28 @@ __device__ __forceinline__ void step_ll_rebuild(const DP &P, const BlockCtx &B, 
         h.ownm = (lane < P.ccap) ? cmem[(size_t)c * P.ccap + lane] : 0;
     };
     const int cstride = B.nb * wpb;
+    // cell c = (x * ncy + y) * ncz + z; the walk adds the stride's own
+    // coordinates with carries instead of dividing for every cell
+    const int ncz = P.ncz, ncy = P.ncy;
+    const int c_first = __builtin_amdgcn_readfirstlane(B.bid * wpb + wid);
+    const int stz = cstride % ncz, sty = (cstride / ncz) % ncy, stx = cstride / (ncz * ncy);
+    int hz = c_first % ncz, hy = (c_first / ncz) % ncy, hx = c_first / (ncz * ncy);   // cell of the next head
     CellHead nxt;
-    if (B.bid * wpb + wid < P.ncells) head(B.bid * wpb + wid, nxt);
-    for (int c = B.bid * wpb + wid; c < P.ncells; c += cstride) {
+    if (c_first < P.ncells) head(c_first, hx, hy, hz, nxt);
+    for (int c = c_first; c < P.ncells; c += cstride) {
         const CellHead cur = nxt;
-        if (c + cstride < P.ncells) head(c + cstride, nxt);
+        hz += stz;
+        hy += sty;
+        hx += stx;
+        if (hz >= ncz) {
+            hz -= ncz;
+            ++hy;
+        }
+        if (hy >= ncy) {
+            hy -= ncy;
+            ++hx;
+        }
+        if (c + cstride < P.ncells) head(c + cstride, hx, hy, hz, nxt);
         const int nca = cur.nca, ncb = cur.ncb, ownm = cur.ownm;
         const int na = min(cur.na, P.ccap);
         const int nb = (sb < 125) ? min(cur.nb, P.ccap) : 0;
